@@ -1,0 +1,16 @@
+"""CB-SpMV for AMD Instinct MI355X (gfx950): Python host side.
+
+The compute path is libcbspmv.so (hand-written HIP kernels behind a C ABI,
+include/cbspmv.h).  This package binds it with ctypes and uses torch only for
+device memory and streams.
+"""
+from ._lib import CbspmvError, LIB_PATH, lib  # noqa: F401
+from .plan import ARRAY_NAMES, Plan, make_opts  # noqa: F401
+from .reference_api import (  # noqa: F401
+    GLOBAL_BLOCK,
+    PROGRAM_METRICS,
+    cb_spmv,
+    coo2block_comp,
+    coo2block_gather,
+    print_benchmark_report,
+)

@@ -1,0 +1,164 @@
+"""Host-side plan object over the C ABI.
+
+A Plan is the GLOBAL_BLOCK of the reference (cb-spmv-dcu/src/format.h:62)
+plus its device slabs: `Plan(m, n, row, col, val)` runs the builder
+(coo2block_gather, cb-spmv-dcu/src/coo2block.h:628), `plan.spmv(x, y)` is one
+device SpMV on torch tensors, and `plan.bench(x)` follows cb_spmv's protocol
+(cb-spmv-dcu/src/cb-spmv-dcu.hpp:1088).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+_EXPORT_DTYPES = {
+    "type_per_blk": np.uint8,
+    "segment_direct": np.uint8,
+    "segment_uniform": np.uint8,
+    "block_row_uniform": np.uint8,
+    "image": np.uint8,
+    "pl_val": np.float64,
+    "stream_key": np.uint32,
+    "stream_val": np.float64,
+}
+ARRAY_NAMES = [
+    "gb_row_idx", "gb_col_idx", "gb_row_ptr", "nnz_per_blk", "type_per_blk",
+    "segment_rows", "segment_begin", "segment_end", "segment_direct", "segment_uniform",
+    "block_row_uniform", "flat_row_ptr", "flat_col_idx", "pattern_row_ptr",
+    "pattern_col_idx", "pattern_output_ptr", "pattern_output_rows", "block_cols",
+    "block_cols_offset", "bia_nnz", "bia_origin", "bia_end", "pl_idx", "pl_val",
+]
+
+_DTYPE_CODES = {"f64": _lib.F64, "f32": _lib.F32, "bf16": _lib.BF16}
+_SCHEDULES = {"auto": _lib.SCHED_AUTO, "blocked": _lib.SCHED_BLOCKED, "stream": _lib.SCHED_STREAM}
+
+
+def make_opts(block_size=16, waves=4, row_owned=True, pattern_wave=True, force_coo64=False,
+              dense_sixteenths=15, builder="gather", dtype="f64", compensated=False,
+              schedule="auto", item_nnz=0):
+    o = _lib.Opts()
+    lib().cbspmv_default_opts(ctypes.byref(o))
+    o.block_size = int(block_size)
+    o.waves_per_block = int(waves)
+    o.row_owned = int(bool(row_owned))
+    o.pattern_wave = int(bool(pattern_wave))
+    o.force_coo64 = int(bool(force_coo64))
+    o.dense_sixteenths = int(dense_sixteenths)
+    o.builder = _lib.BUILDER_COMP if builder == "comp" else _lib.BUILDER_GATHER
+    o.value_dtype = _DTYPE_CODES[dtype]
+    o.accumulate = _lib.ACC_COMPENSATED if compensated else _lib.ACC_NATIVE
+    o.schedule = _SCHEDULES[schedule]
+    o.stream_item_nnz = int(item_nnz)
+    return o
+
+
+def _as_i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class Plan:
+    """A CB plan built from a row-major COO matrix (m x n)."""
+
+    def __init__(self, m, n, row, col, val, **opts):
+        row, col = _as_i32(row), _as_i32(col)
+        val = np.ascontiguousarray(np.asarray(val, dtype=np.float64))
+        if not (row.shape == col.shape == val.shape) or row.ndim != 1:
+            raise ValueError("row, col and val must be 1-D arrays of equal length")
+        self.m, self.n, self.nnz = int(m), int(n), int(row.size)
+        self.dtype = opts.get("dtype", "f64")
+        o = make_opts(**opts)
+        h = ctypes.c_void_p()
+        check(lib().cbspmv_build(self.m, self.n, self.nnz, _ptr(row, ctypes.c_int), _ptr(col, ctypes.c_int),
+                                 _ptr(val, ctypes.c_double), ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self._uploaded = False
+
+    # ---- lifecycle
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().cbspmv_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ---- host-side inspection
+    def stats(self):
+        s = _lib.Stats()
+        check(lib().cbspmv_plan_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def export(self, name):
+        """Returns the named plan array as a numpy array (see cbspmv.h)."""
+        cnt = ctypes.c_size_t(0)
+        check(lib().cbspmv_export(self._h, name.encode(), None, ctypes.byref(cnt)))
+        dt = _EXPORT_DTYPES.get(name, np.int32)
+        out = np.empty(cnt.value, dtype=dt)
+        if cnt.value:
+            check(lib().cbspmv_export(self._h, name.encode(), out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(cnt)))
+        return out
+
+    def layout_offset(self, name):
+        return int(lib().cbspmv_layout_offset(self._h, name.encode()))
+
+    # ---- device execution
+    @property
+    def vector_dtype(self):
+        return torch.float64 if self.dtype == "f64" else torch.float32
+
+    def upload(self, stream=None):
+        check(lib().cbspmv_upload(self._h, _stream_handle(stream)))
+        self._uploaded = True
+
+    def set_schedule(self, name):
+        check(lib().cbspmv_set_schedule(self._h, _SCHEDULES[name]))
+
+    def schedule(self):
+        code = lib().cbspmv_get_schedule(self._h)
+        return {v: k for k, v in _SCHEDULES.items()}[code]
+
+    def _check_vec(self, t, length, name):
+        if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+            raise TypeError(f"{name} must be a CUDA (HIP) tensor")
+        if t.dtype != self.vector_dtype or not t.is_contiguous() or t.numel() < length:
+            raise ValueError(f"{name} must be a contiguous {self.vector_dtype} tensor of >= {length} elements")
+
+    def spmv(self, x, y=None, stream=None):
+        """y = A x on the device (torch tensors); returns y."""
+        if not self._uploaded:
+            self.upload(stream)
+        self._check_vec(x, self.n, "x")
+        if y is None:
+            y = torch.empty(self.m, dtype=self.vector_dtype, device=x.device)
+        self._check_vec(y, self.m, "y")
+        check(lib().cbspmv_spmv(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                _stream_handle(stream)))
+        return y
+
+    def bench(self, x):
+        """cb_spmv() protocol on host FP64 x: returns (y, runtime_ms, gflops)."""
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+        y = np.zeros(self.m, dtype=np.float64)
+        met = _lib.Metrics()
+        check(lib().cbspmv_bench(self._h, _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double), ctypes.byref(met)))
+        self._uploaded = True
+        return y, met.runtime, met.gflops

@@ -1,0 +1,110 @@
+// gfx950 (CDNA4, wave64) device code shared by the CB-SpMV schedules.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "cb_plan.hpp"
+
+namespace cbs {
+namespace dev {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+// Lanes 0..lane inclusive.
+__device__ __forceinline__ uint64_t lanes_le(int lane) {
+    return lane == 63 ? ~0ull : ((2ull << lane) - 1);
+}
+
+// ---- value types ------------------------------------------------------------
+// BF16 values are stored as the upper half of an FP32 bit pattern.
+struct bf16_t { uint16_t bits; };
+
+__device__ __forceinline__ double widen(double v) { return v; }
+__device__ __forceinline__ float widen(float v) { return v; }
+__device__ __forceinline__ float widen(bf16_t v) { return __uint_as_float((uint32_t)v.bits << 16); }
+
+template <typename T>
+__device__ __forceinline__ T stream_load(const T* p) { return __builtin_nontemporal_load(p); }
+template <>
+__device__ __forceinline__ bf16_t stream_load(const bf16_t* p) {
+    bf16_t r;
+    r.bits = __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
+    return r;
+}
+
+// ---- wave shuffles (ds_bpermute) and DPP -------------------------------------
+template <typename T>
+__device__ __forceinline__ T shfl_up(T v, int d) { return __shfl_up(v, d, kWave); }
+template <typename T>
+__device__ __forceinline__ T shfl_down(T v, int d) { return __shfl_down(v, d, kWave); }
+template <typename T>
+__device__ __forceinline__ T shfl_xor(T v, int d) { return __shfl_xor(v, d, kWave); }
+
+// DPP row_shr:n on a 32-bit lane value (lanes shifted out read 0).
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo = dpp32<CTRL>((uint32_t)b), hi = dpp32<CTRL>((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) { return __uint_as_float(dpp32<CTRL>(__float_as_uint(v))); }
+
+// Sum over groups of G consecutive lanes (G = 2, 4, 8 via DPP row_shr, larger
+// via shuffles); the total lands in the group's LAST lane.
+template <int G, typename T>
+__device__ __forceinline__ T group_sum_to_last(T v) {
+    if constexpr (G >= 2) v += dpp<0x111>(v);
+    if constexpr (G >= 4) v += dpp<0x112>(v);
+    if constexpr (G >= 8) v += dpp<0x114>(v);
+    if constexpr (G >= 16) v += dpp<0x118>(v);
+    if constexpr (G >= 32) {
+        // across 16-lane rows: lanes shifted by 16 (shuffle; DPP rows are 16 wide)
+        const T o = shfl_up(v, 16);
+        if ((lane_id() & 31) >= 16) v += o;
+    }
+    return v;
+}
+
+// Segmented inclusive scan over the wave for non-decreasing segment keys:
+// every lane ends with the sum of its run prefix; run-end lanes hold run sums.
+template <typename T>
+__device__ __forceinline__ T seg_scan_sorted(T v, int seg) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const T pv = shfl_up(v, d);
+        const int ps = shfl_up(seg, d);
+        if (lane >= d && ps == seg) v += pv;
+    }
+    return v;
+}
+
+// Error-free transforms for the compensated (two-float) accumulation:
+// Knuth's TwoSum and the FMA TwoProduct (Ogita, Rump, Oishi, "Accurate Sum
+// and Dot Product", SIAM J. Sci. Comput. 26(6), 2005).
+struct f2 { float hi, lo; };
+__device__ __forceinline__ f2 two_sum(float a, float b) {
+    const float s = a + b;
+    const float bb = s - a;
+    const float e = (a - (s - bb)) + (b - bb);
+    return {s, e};
+}
+__device__ __forceinline__ f2 add_f2(f2 a, f2 b) {
+    const f2 s = two_sum(a.hi, b.hi);
+    return {s.hi, s.lo + (a.lo + b.lo)};
+}
+__device__ __forceinline__ f2 two_prod(float a, float b) {
+    const float p = a * b;
+    return {p, __builtin_fmaf(a, b, -p)};
+}
+
+}  // namespace dev
+}  // namespace cbs

@@ -1,0 +1,390 @@
+// Packers: the canonical slab (the reference's device layout, byte for byte)
+// and the stream slab (the transcoded layout of the stream schedule).
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include <omp.h>
+
+#include "cb_plan.hpp"
+
+namespace cbs {
+
+// ---- canonical slab: cb-spmv-dcu/src/cb-spmv-dcu.hpp:1100-1348 -------------
+// Regions in order, each start aligned to its element type: block rows/cols,
+// RO segment arrays, nnz and type per block, pattern arrays, gather restore
+// map, block address table, per-block payloads (final block order), x, y, RO
+// output epochs.  The address table holds byte offsets from the slab base
+// (0 = no payload) where the reference stores device pointers, so the image
+// does not depend on where it is uploaded.
+void pack_canonical(Plan& p) {
+    if (p.image_ready) return;
+    const int bs = p.o.bs;
+    bool unit = true;
+    for (int g = 0; g < p.nnzb && unit; ++g) {
+        const int o = p.bia_origin[g];
+        if (p.nnzpb[g] <= 0) continue;
+        const int t = p.type[g] & kTypeMask;
+        if (t == kCOO) {
+            for (int q = 0; q < p.coo_count[o]; ++q) unit = unit && p.ent_val[p.pl_start[o] + q] == 1.0;
+        } else if (t == kCSR) {
+            for (int q = 0; q < p.raw_nnz[o]; ++q) unit = unit && p.ent_val[p.pl_start[o] + q] == 1.0;
+        } else {
+            const double* d = p.dense_val.data() + (size_t)p.dense_slot[o] * bs * bs;
+            for (int q = 0; q < bs * bs; ++q) unit = unit && (d[q] == 0.0 || d[q] == 1.0);
+        }
+    }
+    p.unit_values = unit;
+
+    size_t cur = 0;
+    auto take = [&cur](size_t bytes, size_t al) { cur = align_up(cur, al); const size_t at = cur; cur += bytes; return at; };
+    const size_t nb = (size_t)p.nnzb, S = p.seg_rows.size();
+    auto& off = p.off;
+    off.clear();
+    off["gb_row_idx"] = take(nb * 4, 4);
+    off["gb_col_idx"] = take(nb * 4, 4);
+    if (p.o.row_owned) {
+        off["segment_rows"] = take(S * 4, 4);
+        off["segment_begin"] = take(S * 4, 4);
+        off["segment_end"] = take(S * 4, 4);
+        off["segment_direct"] = take(S, 1);
+        off["segment_uniform"] = take(S, 1);
+    }
+    off["nnz_per_blk"] = take(nb * 4, 4);
+    off["type_per_blk"] = take(nb, 1);
+    if (p.o.pattern_wave) {
+        off["pattern_row_ptr"] = take(p.pat_row_ptr.size() * 4, 4);
+        off["pattern_col_idx"] = take(p.pat_col.size() * 4, 4);
+        off["pattern_output_ptr"] = take(p.pat_out_ptr.size() * 4, 4);
+        off["pattern_output_rows"] = take(p.pat_out_rows.size() * 4, 4);
+    }
+    if (p.gather) {
+        off["block_cols"] = take(p.block_cols.size() * 4, 4);
+        off["block_cols_offset"] = take(p.block_cols_off.size() * 4, 4);
+    }
+    off["block_addr"] = take(nb * 8, 8);
+    const size_t ob = orig_index_bytes(bs), mb = dense_mask_bytes(bs);
+    std::vector<size_t> boff(nb, 0);
+    for (size_t g = 0; g < nb; ++g) {
+        if (p.nnzpb[g] <= 0) continue;
+        const int o = p.bia_origin[g];
+        const int t = p.type[g] & kTypeMask;
+        if (t == kCOO) {
+            const size_t k = (size_t)p.coo_count[o];
+            const size_t ib = k * (p.compact ? 2 : 4);
+            boff[g] = take(unit ? ib : align_up(ib, 8) + k * 8, unit ? 4 : 8);
+        } else if (t == kCSR) {
+            const size_t k = (size_t)p.raw_nnz[o];
+            const size_t ib = (bs + 1) * ob + k * ob;
+            boff[g] = take(unit ? ib : align_up(ib, 8) + k * 8, unit ? ob : 8);
+        } else {
+            boff[g] = take(unit ? bs * mb : (size_t)bs * bs * 8, unit ? mb : 8);
+        }
+    }
+    off["x"] = take((size_t)p.gb_n * bs * 8, 8);
+    off["y"] = take((size_t)p.gb_m * bs * 8, 8);
+    if (p.o.row_owned) off["output_epochs"] = take((size_t)p.gb_m * 4, 4);
+    off["total"] = cur;
+
+    p.image.assign(off["y"], 0);
+    auto put = [&p](size_t at, const void* src, size_t bytes) { if (bytes) std::memcpy(p.image.data() + at, src, bytes); };
+    put(off["gb_row_idx"], p.brow.data(), nb * 4);
+    put(off["gb_col_idx"], p.bcol.data(), nb * 4);
+    if (p.o.row_owned) {
+        put(off["segment_rows"], p.seg_rows.data(), S * 4);
+        put(off["segment_begin"], p.seg_begin.data(), S * 4);
+        put(off["segment_end"], p.seg_end.data(), S * 4);
+        put(off["segment_direct"], p.seg_direct.data(), S);
+        put(off["segment_uniform"], p.seg_uniform.data(), S);
+    }
+    put(off["nnz_per_blk"], p.nnzpb.data(), nb * 4);
+    put(off["type_per_blk"], p.type.data(), nb);
+    if (p.o.pattern_wave) {
+        put(off["pattern_row_ptr"], p.pat_row_ptr.data(), p.pat_row_ptr.size() * 4);
+        put(off["pattern_col_idx"], p.pat_col.data(), p.pat_col.size() * 4);
+        put(off["pattern_output_ptr"], p.pat_out_ptr.data(), p.pat_out_ptr.size() * 4);
+        put(off["pattern_output_rows"], p.pat_out_rows.data(), p.pat_out_rows.size() * 4);
+    }
+    if (p.gather) {
+        put(off["block_cols"], p.block_cols.data(), p.block_cols.size() * 4);
+        put(off["block_cols_offset"], p.block_cols_off.data(), p.block_cols_off.size() * 4);
+    }
+    std::vector<uint64_t> addr(nb, 0);
+    for (size_t g = 0; g < nb; ++g) if (p.nnzpb[g] > 0) addr[g] = boff[g];
+    put(off["block_addr"], addr.data(), nb * 8);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (long long gg = 0; gg < (long long)nb; ++gg) {
+        const size_t g = (size_t)gg;
+        if (p.nnzpb[g] <= 0) continue;
+        const int o = p.bia_origin[g];
+        const int t = p.type[g] & kTypeMask;
+        uint8_t* dst = p.image.data() + boff[g];
+        if (t == kCOO) {
+            const size_t k = (size_t)p.coo_count[o];
+            const uint32_t* idx = p.coo_idx.data() + p.pl_start[o];
+            if (p.compact) {
+                for (size_t q = 0; q < k; ++q) { const uint16_t w = (uint16_t)idx[q]; std::memcpy(dst + 2 * q, &w, 2); }
+            } else {
+                std::memcpy(dst, idx, k * 4);
+            }
+            if (!unit) std::memcpy(dst + align_up(k * (p.compact ? 2 : 4), 8), p.ent_val.data() + p.pl_start[o], k * 8);
+        } else if (t == kCSR) {
+            const size_t k = (size_t)p.raw_nnz[o];
+            const int* rp = p.csr_rp.data() + p.csr_rp_start[o];
+            const int* cc = p.csr_col.data() + p.pl_start[o];
+            auto put_orig = [&](size_t pos, int v) {
+                if (ob == 1) dst[pos] = (uint8_t)v;
+                else { const uint16_t w = (uint16_t)v; std::memcpy(dst + pos * 2, &w, 2); }
+            };
+            for (int r = 0; r <= bs; ++r) put_orig((size_t)r, rp[r]);
+            for (size_t q = 0; q < k; ++q) put_orig(bs + 1 + q, cc[q]);
+            if (!unit) std::memcpy(dst + align_up((bs + 1 + k) * ob, 8), p.ent_val.data() + p.pl_start[o], k * 8);
+        } else {
+            const double* d = p.dense_val.data() + (size_t)p.dense_slot[o] * bs * bs;
+            if (unit) {
+                for (int r = 0; r < bs; ++r) {
+                    uint64_t mask = 0;
+                    for (int c = 0; c < bs; ++c) if (d[r * bs + c] != 0.0) mask |= 1ull << c;
+                    std::memcpy(dst + (size_t)r * mb, &mask, mb);
+                }
+            } else {
+                std::memcpy(dst, d, (size_t)bs * bs * 8);
+            }
+        }
+    }
+    p.image_ready = true;
+}
+
+// ---- stream slab -------------------------------------------------------------
+// Every block of a strip (block row) contributes entries (column, row-in-strip,
+// value): COO blocks their merged entries, CSR blocks their entries, DENSE
+// blocks all bs*bs positions.  Columns are original columns: a gathered
+// block's local column goes through the restore map once, here, instead of
+// once per SpMV.  Inside a strip, entries are ordered by row, then block
+// order, then in-block order, so that the rows of a strip are contiguous runs
+// and a wave reduces them with a segmented scan instead of atomics.
+void build_stream(Plan& p) {
+    if (p.stream_ready) return;
+    const int bs = p.o.bs, bcb = coord_bits(bs);
+    int cb = 1;
+    while ((1ll << cb) < (long long)p.n) ++cb;
+    p.col_bits = cb;
+
+    // blocks of each strip in final block order (pad blocks skipped)
+    std::vector<int> sb_ptr(p.gb_m + 1, 0), sb_list;
+    for (int g = 0; g < p.nnzb; ++g) if (p.nnzpb[g] > 0) sb_ptr[p.brow[g] + 1]++;
+    for (int b = 0; b < p.gb_m; ++b) sb_ptr[b + 1] += sb_ptr[b];
+    sb_list.assign(sb_ptr[p.gb_m], 0);
+    {
+        std::vector<int> c(sb_ptr.begin(), sb_ptr.end() - 1);
+        for (int g = 0; g < p.nnzb; ++g) if (p.nnzpb[g] > 0) sb_list[c[p.brow[g]]++] = g;
+    }
+    // usable columns of a block: DENSE positions past the gathered column
+    // count (or past n) hold no entry and are skipped, as the reference's
+    // kernels skip them (cb-spmv-dcu.hpp:962, :1018, :1034)
+    auto block_cols_used = [&](int g) {
+        const int b = p.brow[g];
+        const int ncols = p.gather ? (p.block_cols_off[b + 1] - p.block_cols_off[b]) : p.n;
+        return std::max(0, std::min(bs, ncols - p.bcol[g] * bs));
+    };
+    auto block_entries = [&](int g) -> int64_t {
+        const int o = p.bia_origin[g];
+        const int t = p.type[g] & kTypeMask;
+        return t == kCOO ? p.coo_count[o] : t == kCSR ? p.raw_nnz[o] : (int64_t)bs * block_cols_used(g);
+    };
+    std::vector<int64_t> strip_ptr(p.gb_m + 1, 0);
+    for (int b = 0; b < p.gb_m; ++b) {
+        int64_t c = 0;
+        for (int q = sb_ptr[b]; q < sb_ptr[b + 1]; ++q) c += block_entries(sb_list[q]);
+        strip_ptr[b + 1] = strip_ptr[b] + c;
+    }
+    const int64_t total = strip_ptr[p.gb_m];
+    p.s_key.assign(total, 0);
+    p.s_val.assign(total, 0.0);
+
+    const uint32_t rmask = (1u << bcb) - 1;
+#pragma omp parallel
+    {
+        std::vector<int64_t> row_cnt(bs + 1);
+        std::vector<uint32_t> tk;
+        std::vector<double> tv;
+        std::vector<int> tr;
+#pragma omp for schedule(dynamic, 64)
+        for (int b = 0; b < p.gb_m; ++b) {
+            tk.clear(); tv.clear(); tr.clear();
+            const int base_col_off = p.gather ? p.block_cols_off[b] : 0;
+            for (int q = sb_ptr[b]; q < sb_ptr[b + 1]; ++q) {
+                const int g = sb_list[q];
+                const int o = p.bia_origin[g];
+                const int t = p.type[g] & kTypeMask;
+                auto orig_col = [&](int lc) -> int {
+                    const int wc = p.bcol[g] * bs + lc;
+                    return p.gather ? p.block_cols[base_col_off + wc] : wc;
+                };
+                if (t == kCOO) {
+                    const uint32_t* idx = p.coo_idx.data() + p.pl_start[o];
+                    const double* v = p.ent_val.data() + p.pl_start[o];
+                    for (int k = 0; k < p.coo_count[o]; ++k) {
+                        int r, c;
+                        if (p.compact) { r = (int)((idx[k] >> 12) & rmask); c = (int)(idx[k] & 0xfff); }
+                        else if (p.direct) { r = (int)((idx[k] >> 16) & rmask); c = (int)(idx[k] & 0xffff); }
+                        else { r = (int)((idx[k] >> bcb) & rmask); c = orig_col((int)(idx[k] & rmask)); }
+                        tr.push_back(r); tk.push_back((uint32_t)c); tv.push_back(v[k]);
+                    }
+                } else if (t == kCSR) {
+                    const int* rp = p.csr_rp.data() + p.csr_rp_start[o];
+                    const int* cc = p.csr_col.data() + p.pl_start[o];
+                    const double* v = p.ent_val.data() + p.pl_start[o];
+                    for (int r = 0; r < bs; ++r)
+                        for (int k = rp[r]; k < rp[r + 1]; ++k) { tr.push_back(r); tk.push_back((uint32_t)orig_col(cc[k])); tv.push_back(v[k]); }
+                } else {
+                    const double* d = p.dense_val.data() + (size_t)p.dense_slot[o] * bs * bs;
+                    const int used = block_cols_used(g);
+                    for (int r = 0; r < bs; ++r)
+                        for (int c = 0; c < used; ++c) { tr.push_back(r); tk.push_back((uint32_t)orig_col(c)); tv.push_back(d[r * bs + c]); }
+                }
+            }
+            // stable counting sort by row-in-strip
+            std::fill(row_cnt.begin(), row_cnt.end(), 0);
+            for (int r : tr) row_cnt[r + 1]++;
+            for (int r = 0; r < bs; ++r) row_cnt[r + 1] += row_cnt[r];
+            const int64_t base = strip_ptr[b];
+            for (size_t k = 0; k < tr.size(); ++k) {
+                const int64_t at = base + row_cnt[tr[k]]++;
+                p.s_key[at] = tk[k] | ((uint32_t)tr[k] << cb);
+                p.s_val[at] = tv[k];
+            }
+            if (!tr.empty()) p.s_key[base] |= 0x80000000u;   // strip-start flag
+        }
+    }
+
+    // ---- wave items: the entry stream cut into pieces of about `target`
+    // entries, each touching at most kMaxItemStrips strips.  A cut that would
+    // leave less than a quarter item on either side of a strip boundary moves
+    // to that boundary.
+    int64_t target = p.o.item_nnz > 0 ? p.o.item_nnz : 1024;
+    target = std::max<int64_t>(64, std::min<int64_t>(target, 1 << 22));
+    p.strip_ids.clear();
+    p.empty_strips.clear();
+    std::vector<int64_t> nz_ptr;
+    for (int b = 0; b < p.gb_m; ++b) {
+        if (strip_ptr[b + 1] > strip_ptr[b]) { p.strip_ids.push_back((uint32_t)b); nz_ptr.push_back(strip_ptr[b]); }
+        else p.empty_strips.push_back((uint32_t)b);
+    }
+    const int64_t NS = (int64_t)p.strip_ids.size();
+    nz_ptr.push_back(total);
+    p.items.clear();
+    int64_t e = 0, s = 0;
+    while (e < total) {
+        StreamItem it{};
+        it.e_begin = (uint32_t)e;
+        it.s_begin = (uint32_t)s;
+        it.first_partial = e > nz_ptr[s];
+        const int64_t s_cap = std::min<int64_t>(NS - 1, s + kMaxItemStrips - 1);
+        int64_t stop = std::min(std::min(total, e + target), nz_ptr[s_cap + 1]);
+        int64_t s_end = s;
+        while (nz_ptr[s_end + 1] < stop) ++s_end;
+        if (stop < nz_ptr[s_end + 1]) {
+            const int64_t head = stop - nz_ptr[s_end], tail = nz_ptr[s_end + 1] - stop;
+            if (tail < target / 4) stop = nz_ptr[s_end + 1];              // take the short rest
+            else if (s_end > s && head < target / 4) { stop = nz_ptr[s_end]; --s_end; }  // leave the strip
+        }
+        it.e_end = (uint32_t)stop;
+        it.n_strips = (uint16_t)(s_end - s + 1);
+        it.last_partial = stop < nz_ptr[s_end + 1];
+        p.items.push_back(it);
+        e = stop;
+        s = it.last_partial ? s_end : s_end + 1;
+    }
+    // split strips: item i opens one with its last strip, interior items hold
+    // only that strip, and the closing item holds it as its first strip
+    const size_t NI = p.items.size();
+    p.splits.clear();
+    p.item_split.assign(NI, 0xffffffffu);
+    p.item_split_first.assign(NI, 0xffffffffu);
+    p.partial_slots = 0;
+    for (size_t i = 0; i < NI; ++i) {
+        const StreamItem& it = p.items[i];
+        if (!it.last_partial) continue;
+        const uint32_t sidx = it.s_begin + it.n_strips - 1;
+        size_t j = i + 1;
+        while (p.items[j].last_partial && p.items[j].n_strips == 1) ++j;   // interior pieces
+        StreamSplit sp{};
+        sp.strip = p.strip_ids[sidx];
+        sp.first_item = (uint32_t)i;
+        sp.pieces = (uint32_t)(j - i + 1);
+        sp.partial_base = p.partial_slots;
+        p.partial_slots += sp.pieces;
+        const uint32_t id = (uint32_t)p.splits.size();
+        p.splits.push_back(sp);
+        p.item_split[i] = id;
+        for (size_t q = i + 1; q < j; ++q) { p.item_split_first[q] = id; p.item_split[q] = id; }
+        p.item_split_first[j] = id;
+        i = j - 1;   // item j may open the next split with its own last strip
+    }
+    p.stream_ready = true;
+}
+
+// ---- parity export -----------------------------------------------------------
+namespace {
+template <typename T>
+long copy_out(const std::vector<T>& v, void* dst, size_t cap) {
+    if (dst && cap >= v.size() && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(T));
+    return (long)v.size();
+}
+}  // namespace
+
+long export_array(const Plan& p, const std::string& s, void* dst, size_t cap, int& status) {
+    status = CBSPMV_OK;
+    if (s == "gb_row_idx") return copy_out(p.brow, dst, cap);
+    if (s == "gb_col_idx") return copy_out(p.bcol, dst, cap);
+    if (s == "gb_row_ptr") return copy_out(p.row_ptr, dst, cap);
+    if (s == "nnz_per_blk") return copy_out(p.nnzpb, dst, cap);
+    if (s == "type_per_blk") return copy_out(p.type, dst, cap);
+    if (s == "segment_rows") return copy_out(p.seg_rows, dst, cap);
+    if (s == "segment_begin") return copy_out(p.seg_begin, dst, cap);
+    if (s == "segment_end") return copy_out(p.seg_end, dst, cap);
+    if (s == "segment_direct") return copy_out(p.seg_direct, dst, cap);
+    if (s == "segment_uniform") return copy_out(p.seg_uniform, dst, cap);
+    if (s == "block_row_uniform") return copy_out(p.row_uniform, dst, cap);
+    if (s == "flat_row_ptr") return copy_out(p.flat_row_ptr, dst, cap);
+    if (s == "flat_col_idx") return copy_out(p.flat_col, dst, cap);
+    if (s == "pattern_row_ptr") return copy_out(p.pat_row_ptr, dst, cap);
+    if (s == "pattern_col_idx") return copy_out(p.pat_col, dst, cap);
+    if (s == "pattern_output_ptr") return copy_out(p.pat_out_ptr, dst, cap);
+    if (s == "pattern_output_rows") return copy_out(p.pat_out_rows, dst, cap);
+    if (s == "block_cols") return copy_out(p.block_cols, dst, cap);
+    if (s == "block_cols_offset") return copy_out(p.block_cols_off, dst, cap);
+    if (s == "bia_nnz") return copy_out(p.bia_nnz, dst, cap);
+    if (s == "bia_origin") return copy_out(p.bia_origin, dst, cap);
+    if (s == "bia_end") return copy_out(p.bia_end, dst, cap);
+    if (s == "image") return copy_out(p.image, dst, cap);
+    if (s == "stream_key") return copy_out(p.s_key, dst, cap);
+    if (s == "stream_val") return copy_out(p.s_val, dst, cap);
+    if (s == "pl_idx" || s == "pl_val") {
+        const int bs = p.o.bs;
+        std::vector<int32_t> idx;
+        std::vector<double> val;
+        for (int g = 0; g < p.nnzb; ++g) {
+            if (p.nnzpb[g] <= 0) continue;
+            const int o = p.bia_origin[g];
+            const int t = p.type[g] & kTypeMask;
+            if (t == kCOO) {
+                for (int q = 0; q < p.coo_count[o]; ++q) idx.push_back((int32_t)p.coo_idx[p.pl_start[o] + q]);
+                val.insert(val.end(), p.ent_val.begin() + p.pl_start[o], p.ent_val.begin() + p.pl_start[o] + p.coo_count[o]);
+            } else if (t == kCSR) {
+                idx.insert(idx.end(), p.csr_rp.begin() + p.csr_rp_start[o], p.csr_rp.begin() + p.csr_rp_start[o] + bs + 1);
+                idx.insert(idx.end(), p.csr_col.begin() + p.pl_start[o], p.csr_col.begin() + p.pl_start[o] + p.raw_nnz[o]);
+                val.insert(val.end(), p.ent_val.begin() + p.pl_start[o], p.ent_val.begin() + p.pl_start[o] + p.raw_nnz[o]);
+            } else {
+                const double* d = p.dense_val.data() + (size_t)p.dense_slot[o] * bs * bs;
+                val.insert(val.end(), d, d + bs * bs);
+            }
+        }
+        return s == "pl_idx" ? copy_out(idx, dst, cap) : copy_out(val, dst, cap);
+    }
+    status = CBSPMV_ERR_ARG;
+    return -1;
+}
+
+}  // namespace cbs

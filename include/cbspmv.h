@@ -1,0 +1,194 @@
+/*
+ * cbspmv.h - C ABI of the MI355X (gfx950) CB-SpMV library, libcbspmv.so.
+ *
+ * Drop-in for the host/device boundary of the reference CB-SpMV-DCU, which has
+ * no library ABI of its own: its callers use three C++ entry points that this
+ * ABI replaces one for one (reference paths relative to the reference root):
+ *
+ *   GLOBAL_BLOCK(m, n, nnz) + coo2block_gather(row, col, val, gb)
+ *       cb-spmv-dcu/src/format.h:106, cb-spmv-dcu/src/coo2block.h:628
+ *       -> cbspmv_build()
+ *   coo2block_comp(row, col, val, gb)   (the never-gathering builder)
+ *       cb-spmv-dcu/src/coo2block.h:605
+ *       -> cbspmv_build() with opts.builder = CBSPMV_BUILDER_COMP
+ *   PROGRAM_METRICS cb_spmv(const GLOBAL_BLOCK&, const double* x, double* y)
+ *       cb-spmv-dcu/src/cb-spmv-dcu.hpp:1088
+ *       -> cbspmv_bench() (same protocol) or cbspmv_upload() + cbspmv_spmv()
+ *
+ * Conventions: every function returns 0 (CBSPMV_OK) or a negative status and
+ * never exits the process (the reference's CHECK_HIP calls exit(1),
+ * macros.h:17-31).  cbspmv_last_error() returns thread-local text for the last
+ * failure.  Host pointers are borrowed for the duration of a call only.
+ * Device pointers (d_*) are caller-owned device memory.  `stream` is a
+ * hipStream_t passed as void* (NULL = the null stream).  A plan is not
+ * thread-safe; distinct plans are independent.
+ */
+#ifndef CBSPMV_H
+#define CBSPMV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define CBSPMV_OK 0
+#define CBSPMV_ERR_ARG -1          /* null pointer, bad size or option value      */
+#define CBSPMV_ERR_RANGE -2        /* a row/col index outside [0,m) / [0,n)       */
+#define CBSPMV_ERR_ORDER -3        /* COO rows not in row-major (non-decreasing)  */
+#define CBSPMV_ERR_HIP -4          /* a HIP runtime call failed                   */
+#define CBSPMV_ERR_NOMEM -5        /* host or device allocation failed            */
+#define CBSPMV_ERR_STATE -6        /* call out of order (e.g. spmv before upload) */
+#define CBSPMV_ERR_UNSUPPORTED -7  /* option combination not supported            */
+
+/* ---- options ---------------------------------------------------------------
+ * The reference fixes these at compile time (cb-spmv-dcu/src/macros.h:38-97,
+ * Makefile:4-7); here they are per plan.  cbspmv_default_opts() returns the
+ * reference Makefile's build: 16x16 blocks, 4 waves, row-owned segments,
+ * pattern reuse enabled, FP64 values. */
+enum cbspmv_builder { CBSPMV_BUILDER_GATHER = 0, CBSPMV_BUILDER_COMP = 1 };
+enum cbspmv_dtype { CBSPMV_F64 = 0, CBSPMV_F32 = 1, CBSPMV_BF16 = 2 };
+enum cbspmv_accum { CBSPMV_ACC_NATIVE = 0, CBSPMV_ACC_COMPENSATED = 1 };
+enum cbspmv_schedule {
+    CBSPMV_SCHED_AUTO = 0,    /* pick by plan size / autotuner                       */
+    CBSPMV_SCHED_BLOCKED = 1, /* one wave per packed block over the canonical slab   */
+    CBSPMV_SCHED_STREAM = 2   /* nnz-balanced strip stream over the transcoded slab  */
+};
+
+typedef struct cbspmv_opts {
+    int block_size;       /* 2, 4, 8, 16, 32 or 64            (CB_BLOCK_SIZE)       */
+    int waves_per_block;  /* 1..16                            (CB_WAVES_PER_BLOCK)  */
+    int row_owned;        /* 1: row-owned segments, 0: LPT    (CB_ROW_OWNED)        */
+    int pattern_wave;     /* 1: pattern-reuse schedule usable (CB_PATTERN_WAVE)     */
+    int force_coo64;      /* 1: COO up to 64 nnz              (CB_FORCE_COO64)      */
+    int dense_sixteenths; /* DENSE threshold, 1..16           (CB_DENSE_SIXTEENTHS) */
+    int builder;          /* enum cbspmv_builder                                     */
+    int value_dtype;      /* enum cbspmv_dtype: storage type of A's values           */
+    int accumulate;       /* enum cbspmv_accum (F32/BF16 only)                       */
+    int schedule;         /* enum cbspmv_schedule                                    */
+    int stream_item_nnz;  /* stream schedule: target entries per wave item (0=auto)  */
+    int reserved[5];
+} cbspmv_opts;
+
+/* Builder statistics: the GLOBAL_BLOCK scalars (format.h:63-77) plus counts. */
+typedef struct cbspmv_stats {
+    int m, n, nnz;
+    int gb_m, gb_n, gb_nnzb;
+    int gather_flag, direct_coo_columns, compact_coo_indices;
+    int super_sparse_blocks, analyzed_blocks;
+    double super_sparse_ratio;
+    int load_balanced;
+    double load_stddev_before, load_stddev_after;
+    int coo_blocks, csr_blocks, dense_blocks, uniform_blocks;
+    int segments, patterns;
+    int unit_values;
+    long long slab_bytes;      /* canonical packed slab incl. x, y, epochs       */
+    long long stream_bytes;    /* transcoded stream slab read per SpMV           */
+    long long restore_entries; /* |block_cols|                                   */
+    double build_seconds;
+} cbspmv_stats;
+
+/* PROGRAM_METRICS (format.h:185-188): runtime in ms per launch, GFLOPS. */
+typedef struct cbspmv_metrics {
+    double runtime;
+    double gflops;
+} cbspmv_metrics;
+
+typedef struct cbspmv_plan cbspmv_plan;
+
+/* ---- plan lifecycle ---------------------------------------------------------- */
+void cbspmv_default_opts(cbspmv_opts* opts);
+
+/* Builds the CB structure from a row-major COO stream (rows non-decreasing;
+ * within-row order as given; duplicates allowed).  Bit-exact with
+ * coo2block_gather (or coo2block_comp) for the same options.  opts may be NULL
+ * (defaults).  nnz == 0 is allowed (y := 0, as cb_spmv, cb-spmv-dcu.hpp:1091). */
+int cbspmv_build(int m, int n, int nnz, const int* row, const int* col,
+                 const double* val, const cbspmv_opts* opts, cbspmv_plan** plan);
+void cbspmv_destroy(cbspmv_plan* plan);
+int cbspmv_plan_stats(const cbspmv_plan* plan, cbspmv_stats* stats);
+
+/* Parity export.  Copies the named host array into dst when dst != NULL and
+ * *count is large enough; always sets *count to the element count.  Names:
+ *   gb_row_idx gb_col_idx gb_row_ptr nnz_per_blk type_per_blk(u8)
+ *   segment_rows segment_begin segment_end segment_direct(u8) segment_uniform(u8)
+ *   block_row_uniform(u8) flat_row_ptr flat_col_idx pattern_row_ptr
+ *   pattern_col_idx pattern_output_ptr pattern_output_rows block_cols
+ *   block_cols_offset bia_nnz bia_origin bia_end       (int32 unless marked)
+ *   pl_idx (int32) pl_val (f64): payload streams in final block order
+ *   image (u8): canonical packed slab bytes up to the y region, block
+ *               addresses stored as byte offsets (0 = none), x region zero
+ * Returns CBSPMV_ERR_ARG for an unknown name. */
+int cbspmv_export(const cbspmv_plan* plan, const char* name, void* dst, size_t* count);
+/* Byte offset of a slab region in the canonical image ("total" = size). */
+long long cbspmv_layout_offset(const cbspmv_plan* plan, const char* name);
+
+/* ---- device execution (stream-ordered on the caller's stream) --------------- */
+/* Copies the plan's device slabs to the current device (allocates). */
+int cbspmv_upload(cbspmv_plan* plan, void* stream);
+/* y = A x.  d_x has n elements, d_y has m elements, both of the plan's
+ * vector type: double for F64 plans, float for F32/BF16 plans.  y is
+ * overwritten (no accumulation).  No allocation, no host synchronisation. */
+int cbspmv_spmv(cbspmv_plan* plan, const void* d_x, void* d_y, void* stream);
+/* Forces one schedule for the next calls (CBSPMV_SCHED_*); AUTO restores the
+ * plan's choice.  Returns CBSPMV_ERR_UNSUPPORTED if the plan lacks it. */
+int cbspmv_set_schedule(cbspmv_plan* plan, int schedule);
+int cbspmv_get_schedule(const cbspmv_plan* plan);
+
+/* cb_spmv() semantics and protocol (cb-spmv-dcu.hpp:1088-1667): upload,
+ * one validated launch whose y is copied to host y, 100 warm-up and 1000
+ * timed launches between hipEvents; runtime = ms / 1000, GFLOPS = 2 nnz / t.
+ * x, y are host arrays of n and m doubles.  Timings go to *metrics. */
+int cbspmv_bench(cbspmv_plan* plan, const double* x, double* y, cbspmv_metrics* metrics);
+
+/* ---- differentiation (no reference counterpart; extension) ------------------ */
+/* Builds the transposed CB structure once (A^T, same options). */
+int cbspmv_build_transpose(cbspmv_plan* plan);
+/* d_dx = A^T d_g  (n elements). */
+int cbspmv_spmv_transpose(cbspmv_plan* plan, const void* d_g, void* d_dx, void* stream);
+/* d_dvals[k] = g[row[k]] * x[col[k]] for every input COO entry k, in input
+ * order (the blockwise SDDMM over the stored pattern).  Also computes
+ * d_dx = A^T d_g when d_dx != NULL, in the same pass over the transposed plan. */
+int cbspmv_backward(cbspmv_plan* plan, const void* d_g, const void* d_x,
+                    void* d_dx, void* d_dvals, void* stream);
+/* Replaces the values of A (input COO order, plan's value dtype promoted to
+ * the vector type: double for F64 plans, float otherwise) on the device. */
+int cbspmv_set_values(cbspmv_plan* plan, const void* d_vals, void* stream);
+
+/* ---- batching (no reference counterpart; extension) -------------------------- */
+typedef struct cbspmv_batch cbspmv_batch;
+/* Builds one batch from `count` plans already built (borrowed; they must
+ * outlive the batch) and uploads a single descriptor table. */
+int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** batch, void* stream);
+/* Y_i = A_i X_i for every matrix i, with nrhs[i] right-hand sides each:
+ * d_x[i] is n_i x nrhs[i] (column-major, leading dim n_i), d_y[i] is
+ * m_i x nrhs[i].  One persistent launch. */
+int cbspmv_batch_spmv(cbspmv_batch* batch, const void* const* d_x, void* const* d_y,
+                      const int* nrhs, void* stream);
+/* Captures the launch above into a hipGraph bound to these pointers and
+ * replays it; cbspmv_batch_replay() launches the captured graph again. */
+int cbspmv_batch_capture(cbspmv_batch* batch, const void* const* d_x, void* const* d_y,
+                         const int* nrhs, void* stream);
+int cbspmv_batch_replay(cbspmv_batch* batch, void* stream);
+void cbspmv_batch_destroy(cbspmv_batch* batch);
+
+/* Autotuner: builds the candidate plans (block size x waves) for one COO
+ * matrix, times each on the device and returns the fastest, destroying the
+ * others.  `candidates` lists (block_size, waves) pairs; NULL = {16,32}x{4,9}. */
+int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const double* val,
+                    const cbspmv_opts* base, const int* candidates, int ncandidates,
+                    int nrhs, cbspmv_plan** best, double* best_ms, void* stream);
+
+/* ---- diagnostics ---------------------------------------------------------------- */
+const char* cbspmv_last_error(void);
+const char* cbspmv_version(void);
+/* Average duration (ms) of the dominant kernel over the last cbspmv_bench
+ * timed loop, from hipEvents on the plan's stream. */
+double cbspmv_last_kernel_ms(const cbspmv_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CBSPMV_H */
