@@ -43,28 +43,59 @@ __device__ __forceinline__ T shfl_down(T v, int d) { return __shfl_down(v, d, kW
 template <typename T>
 __device__ __forceinline__ T shfl_xor(T v, int d) { return __shfl_xor(v, d, kWave); }
 
-// DPP row_shr:n on a 32-bit lane value (lanes shifted out read 0).
-template <int CTRL>
-__device__ __forceinline__ uint32_t dpp32(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
+// DPP move of a 32-bit lane value; lanes without a source (outside the row,
+// or in rows not selected by ROWMASK) read `old`.
+template <int CTRL, int ROWMASK = 0xf>
+__device__ __forceinline__ uint32_t dpp32(uint32_t v, uint32_t old = 0) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWMASK, 0xf, false);
 }
-template <int CTRL>
+template <int CTRL, int ROWMASK = 0xf>
 __device__ __forceinline__ double dpp(double v) {
-    const uint64_t b = __double_as_longlong(v);
-    const uint32_t lo = dpp32<CTRL>((uint32_t)b), hi = dpp32<CTRL>((uint32_t)(b >> 32));
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = dpp32<CTRL, ROWMASK>((uint32_t)b), hi = dpp32<CTRL, ROWMASK>((uint32_t)(b >> 32));
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
-template <int CTRL>
-__device__ __forceinline__ float dpp(float v) { return __uint_as_float(dpp32<CTRL>(__float_as_uint(v))); }
+template <int CTRL, int ROWMASK = 0xf>
+__device__ __forceinline__ float dpp(float v) { return __uint_as_float(dpp32<CTRL, ROWMASK>(__float_as_uint(v))); }
+
+// DPP controls (GFX9): row_shr:n = 0x110 + n, row_bcast:15 = 0x142,
+// row_bcast:31 = 0x143.
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+
+// One step of a segmented inclusive scan: add the value DPP brings in when it
+// comes from the same segment.  `seg` is non-decreasing along the wave, so an
+// equal key means every lane in between belongs to the same run.
+template <int CTRL, int ROWMASK, typename T>
+__device__ __forceinline__ void seg_step(T& v, int seg) {
+    const T pv = dpp<CTRL, ROWMASK>(v);
+    const int ps = (int)dpp32<CTRL, ROWMASK>((uint32_t)seg, 0xffffffffu);
+    if (ps == seg) v += pv;
+}
+
+// Segmented inclusive scan over the 64 lanes for non-decreasing keys, all on
+// DPP: row_shr 1, 2, 4, 8 inside each 16-lane row, then row_bcast:15 into
+// rows 1 and 3 and row_bcast:31 into rows 2 and 3.  Run-end lanes end with
+// their run's sum.
+template <typename T>
+__device__ __forceinline__ T seg_scan_dpp(T v, int seg) {
+    seg_step<kRowShr1, 0xf>(v, seg);
+    seg_step<kRowShr2, 0xf>(v, seg);
+    seg_step<kRowShr4, 0xf>(v, seg);
+    seg_step<kRowShr8, 0xf>(v, seg);
+    seg_step<kRowBcast15, 0xa>(v, seg);
+    seg_step<kRowBcast31, 0xc>(v, seg);
+    return v;
+}
 
 // Sum over groups of G consecutive lanes (G = 2, 4, 8 via DPP row_shr, larger
 // via shuffles); the total lands in the group's LAST lane.
 template <int G, typename T>
 __device__ __forceinline__ T group_sum_to_last(T v) {
-    if constexpr (G >= 2) v += dpp<0x111>(v);
-    if constexpr (G >= 4) v += dpp<0x112>(v);
-    if constexpr (G >= 8) v += dpp<0x114>(v);
-    if constexpr (G >= 16) v += dpp<0x118>(v);
+    if constexpr (G >= 2) v += dpp<kRowShr1>(v);
+    if constexpr (G >= 4) v += dpp<kRowShr2>(v);
+    if constexpr (G >= 8) v += dpp<kRowShr4>(v);
+    if constexpr (G >= 16) v += dpp<kRowShr8>(v);
     if constexpr (G >= 32) {
         // across 16-lane rows: lanes shifted by 16 (shuffle; DPP rows are 16 wide)
         const T o = shfl_up(v, 16);

@@ -258,12 +258,13 @@ void build_stream(Plan& p) {
         }
     }
 
-    // ---- wave items: the entry stream cut into pieces of about `target`
-    // entries, each touching at most kMaxItemStrips strips.  A cut that would
-    // leave less than a quarter item on either side of a strip boundary moves
-    // to that boundary.
-    int64_t target = p.o.item_nnz > 0 ? p.o.item_nnz : 1024;
+    // ---- wave items: runs of whole strips of about `target` entries (at
+    // most kMaxItemStrips strips).  Only a strip longer than 4 * target is
+    // cut, into pieces of about `target`; those pieces meet through the
+    // partial buffer, which costs device-scope fences, so cuts stay rare.
+    int64_t target = p.o.item_nnz > 0 ? p.o.item_nnz : 2048;
     target = std::max<int64_t>(64, std::min<int64_t>(target, 1 << 22));
+    const int64_t split_len = 4 * target;
     p.strip_ids.clear();
     p.empty_strips.clear();
     std::vector<int64_t> nz_ptr;
@@ -274,27 +275,37 @@ void build_stream(Plan& p) {
     const int64_t NS = (int64_t)p.strip_ids.size();
     nz_ptr.push_back(total);
     p.items.clear();
-    int64_t e = 0, s = 0;
-    while (e < total) {
-        StreamItem it{};
-        it.e_begin = (uint32_t)e;
-        it.s_begin = (uint32_t)s;
-        it.first_partial = e > nz_ptr[s];
-        const int64_t s_cap = std::min<int64_t>(NS - 1, s + kMaxItemStrips - 1);
-        int64_t stop = std::min(std::min(total, e + target), nz_ptr[s_cap + 1]);
-        int64_t s_end = s;
-        while (nz_ptr[s_end + 1] < stop) ++s_end;
-        if (stop < nz_ptr[s_end + 1]) {
-            const int64_t head = stop - nz_ptr[s_end], tail = nz_ptr[s_end + 1] - stop;
-            if (tail < target / 4) stop = nz_ptr[s_end + 1];              // take the short rest
-            else if (s_end > s && head < target / 4) { stop = nz_ptr[s_end]; --s_end; }  // leave the strip
+    int64_t s = 0;
+    while (s < NS) {
+        const int64_t len = nz_ptr[s + 1] - nz_ptr[s];
+        if (len > split_len) {
+            const int64_t pieces = (len + target - 1) / target;
+            for (int64_t q = 0; q < pieces; ++q) {
+                StreamItem it{};
+                it.e_begin = (uint32_t)(nz_ptr[s] + len * q / pieces);
+                it.e_end = (uint32_t)(nz_ptr[s] + len * (q + 1) / pieces);
+                it.s_begin = (uint32_t)s;
+                it.n_strips = 1;
+                it.first_partial = q > 0;
+                it.last_partial = q + 1 < pieces;
+                p.items.push_back(it);
+            }
+            ++s;
+            continue;
         }
-        it.e_end = (uint32_t)stop;
-        it.n_strips = (uint16_t)(s_end - s + 1);
-        it.last_partial = stop < nz_ptr[s_end + 1];
+        StreamItem it{};
+        it.s_begin = (uint32_t)s;
+        it.e_begin = (uint32_t)nz_ptr[s];
+        int64_t sum = 0, s0 = s;
+        while (s < NS && s - s0 < kMaxItemStrips && sum < target && nz_ptr[s + 1] - nz_ptr[s] <= split_len) {
+            sum += nz_ptr[s + 1] - nz_ptr[s];
+            ++s;
+        }
+        it.e_end = (uint32_t)nz_ptr[s];
+        it.n_strips = (uint16_t)(s - s0);
+        it.first_partial = 0;
+        it.last_partial = 0;
         p.items.push_back(it);
-        e = stop;
-        s = it.last_partial ? s_end : s_end + 1;
     }
     // split strips: item i opens one with its last strip, interior items hold
     // only that strip, and the closing item holds it as its first strip
@@ -361,6 +372,17 @@ long export_array(const Plan& p, const std::string& s, void* dst, size_t cap, in
     if (s == "image") return copy_out(p.image, dst, cap);
     if (s == "stream_key") return copy_out(p.s_key, dst, cap);
     if (s == "stream_val") return copy_out(p.s_val, dst, cap);
+    if (s == "stream_strip_ids") return copy_out(p.strip_ids, dst, cap);
+    if (s == "stream_empty_strips") return copy_out(p.empty_strips, dst, cap);
+    if (s == "stream_items") {
+        std::vector<uint32_t> v;
+        for (const StreamItem& it : p.items) {
+            v.push_back(it.e_begin); v.push_back(it.e_end); v.push_back(it.s_begin);
+            v.push_back((uint32_t)it.n_strips | ((uint32_t)it.first_partial << 16) | ((uint32_t)it.last_partial << 24));
+        }
+        return copy_out(v, dst, cap);
+    }
+    if (s == "stream_col_bits") return copy_out(std::vector<int>{p.col_bits}, dst, cap);
     if (s == "pl_idx" || s == "pl_val") {
         const int bs = p.o.bs;
         std::vector<int32_t> idx;

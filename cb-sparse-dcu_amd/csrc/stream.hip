@@ -55,32 +55,24 @@ __global__ __launch_bounds__(kStreamThreads) void stream_spmv(StreamArgs<V, A> a
         }
         __builtin_amdgcn_wave_barrier();
         int kbase = it.first_partial ? 0 : -1;
+        const uint32_t e_last = it.e_end - 1;
         for (uint32_t e0 = it.e_begin; e0 < it.e_end; e0 += kWave * U) {
+            // all loads unconditional (index clamped to the item) so that
+            // the U key/value loads and then the U x gathers issue together
             uint32_t key[U];
             V val[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t e = e0 + u * kWave + lane;
-                if (e < it.e_end) {
-                    key[u] = stream_load(a.key + e);
-                    val[u] = stream_load(a.val + e);
-                } else {
-                    key[u] = 0;
-                    val[u] = V{};
-                }
+                const uint32_t e = min(e0 + u * kWave + lane, e_last);
+                key[u] = stream_load(a.key + e);
+                val[u] = stream_load(a.val + e);
             }
-            Acc prod[U];
+            A xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = a.x[key[u] & cmask];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t e = e0 + u * kWave + lane;
-                const A xv = e < it.e_end ? a.x[key[u] & cmask] : A(0);
-                if constexpr (COMP) prod[u] = two_prod(widen(val[u]), xv);
-                else prod[u] = A(widen(val[u])) * xv;
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t e = e0 + u * kWave + lane;
-                const bool valid = e < it.e_end;
+                const bool valid = e0 + u * kWave + lane <= e_last;
                 const uint64_t starts = __ballot(valid && (key[u] >> 31));
                 const int k = kbase + (int)__popcll(starts & lanes_le(lane));
                 kbase += (int)__popcll(starts);
@@ -89,7 +81,7 @@ __global__ __launch_bounds__(kStreamThreads) void stream_spmv(StreamArgs<V, A> a
                 Acc v;
                 if constexpr (COMP) {
                     // segmented scan on (hi, lo) pairs with TwoSum
-                    v = prod[u];
+                    v = valid ? two_prod(widen(val[u]), xv[u]) : f2{0.f, 0.f};
 #pragma unroll
                     for (int d = 1; d < kWave; d <<= 1) {
                         const float ph = shfl_up(v.hi, d), pl = shfl_up(v.lo, d);
@@ -97,13 +89,13 @@ __global__ __launch_bounds__(kStreamThreads) void stream_spmv(StreamArgs<V, A> a
                         if (lane >= d && ps == seg) v = add_f2(v, f2{ph, pl});
                     }
                 } else {
-                    v = seg_scan_sorted(prod[u], seg);
+                    v = seg_scan_dpp(valid ? A(widen(val[u])) * xv[u] : A(0), seg);
                 }
                 const int nseg = shfl_down(seg, 1);
                 const bool end = valid && (lane == kWave - 1 || nseg != seg);
                 if (end) {
                     if constexpr (COMP) acc[seg] = add_f2(acc[seg], v);
-                    else acc[seg] += v;
+                    else atomicAdd(acc + seg, v);   // ds_add: run ends are unique per chunk
                 }
             }
         }
