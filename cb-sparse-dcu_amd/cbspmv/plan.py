@@ -23,9 +23,11 @@ _EXPORT_DTYPES = {
     "pl_val": np.float64,
     "stream_key": np.uint32,
     "stream_val": np.float64,
-    "stream_strip_ids": np.uint32,
-    "stream_empty_strips": np.uint32,
     "stream_items": np.uint32,
+    "stream_slab": np.uint8,
+    "stream_tiles": np.uint32,
+    "stream_tile_lines": np.uint32,
+    "stream_empty_rows": np.uint32,
 }
 ARRAY_NAMES = [
     "gb_row_idx", "gb_col_idx", "gb_row_ptr", "nnz_per_blk", "type_per_blk",

@@ -57,8 +57,9 @@ int cbspmv_build(int m, int n, int nnz, const int* row, const int* col, const do
         return fail(CBSPMV_ERR_ARG, "unknown value dtype");
     if (o.schedule < CBSPMV_SCHED_AUTO || o.schedule > CBSPMV_SCHED_STREAM)
         return fail(CBSPMV_ERR_ARG, "unknown schedule");
-    if (n > (1 << 30)) return fail(CBSPMV_ERR_UNSUPPORTED, "n above 2^30 is not supported");
-    if (bs > 16 && n > (1 << 26)) return fail(CBSPMV_ERR_UNSUPPORTED, "n above 2^26 needs block_size <= 16");
+    // stream keys hold column + row-in-strip + 2 flag bits in 32 bits
+    if ((long long)n > (1ll << (30 - cbs::coord_bits(bs))))
+        return fail(CBSPMV_ERR_UNSUPPORTED, "n above 2^(30 - log2 block_size) is not supported");
 
     cbspmv_plan* p = new (std::nothrow) cbspmv_plan();
     if (!p) return fail(CBSPMV_ERR_NOMEM, "out of host memory");

@@ -35,28 +35,50 @@ inline size_t orig_index_bytes(int bs) { return bs <= 16 ? 1 : 2; }   // IndexTy
 inline size_t dense_mask_bytes(int bs) { return bs <= 8 ? 1 : bs == 16 ? 2 : bs == 32 ? 4 : 8; }
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) & ~(a - 1); }
 
-// Stream slab (device layout read by the stream schedule).  Within each block
-// row ("strip") the entries of all its blocks are ordered by in-block row,
-// then block order, then in-block order; each entry is one 32-bit key and one
-// value.  key = original column | (row-in-strip << col_bits) | strip-start
-// flag at bit 31.  Work is cut into wave "items" of about item_nnz entries
-// covering at most kMaxItemStrips strips; a strip longer than an item is cut
-// into pieces whose partial rows meet in a small partial buffer.
-struct StreamItem {        // 16 bytes, read once per wave
-    uint32_t e_begin;      // first entry
-    uint32_t e_end;        // one past the last entry
-    uint32_t s_begin;      // first strip (index into strip_ids)
-    uint16_t n_strips;     // strips touched by the item
-    uint8_t first_partial; // first strip continues from the previous item
-    uint8_t last_partial;  // last strip continues into the next item
+// Stream slab (device layout read by the stream schedule).
+//
+// Entries: inside each strip (block row) the entries of all its blocks are
+// ordered by in-block row, then block order, then in-block order, so every
+// matrix row is one contiguous run.  Each entry is a 32-bit key and a value:
+//   bit 31       column is staged in the workgroup's LDS x cache
+//   [cb, 31)     row offset from the item's base row
+//   [0, cb)      original column, or LDS slot when bit 31 is set
+// Work: the stream is cut into wave items of about item_nnz entries at row
+// boundaries (an item's rows span less than 2^(31-cb)); only a row longer
+// than 4 items is cut, and its pieces meet in a small partial buffer through
+// an atomic ticket.  Every item is padded with zero entries (value 0 times
+// the always-zero LDS slot) to a multiple of the entries each lane owns, so
+// lanes are either full or idle.  kItemsPerTile consecutive items form a tile
+// run by one workgroup, which first stages the tile's hot x cache lines (16
+// elements each, lines hit at least twice) into LDS.
+struct StreamItem {        // 16 bytes
+    uint32_t e_begin;      // first entry (a multiple of the lane width)
+    uint32_t e_end;        // one past the last entry (padded)
+    uint32_t row_base;     // row of the first entry
+    uint8_t first_partial; // first run continues a row from the previous item
+    uint8_t last_partial;  // last run continues into the next item
+    uint8_t pad[2];
 };
-constexpr int kMaxItemStrips = 16;
+constexpr int kItemsPerTile = 8;        // waves per stream workgroup
+constexpr int kTileBytes = 80 * 1024;   // LDS x cache per workgroup (2 per CU)
+constexpr int kLineElems = 16;          // x elements per cached line
+constexpr uint32_t kKeyCached = 0x80000000u;
+// LDS x-cache elements per workgroup; the last slot always holds 0
+inline int tile_slots(int dtype) { return kTileBytes / (dtype == CBSPMV_F64 ? 8 : 4); }
+inline int tile_lines(int dtype) { return tile_slots(dtype) / kLineElems - 1; }
+// Entries one lane owns per chunk (a chunk is 64 lanes' worth).  The slab
+// stores each chunk as 16-byte lane planes: the keys (1 or 2 planes), then
+// the values (1 or 2 planes); plane p holds 16 bytes per lane, lane-major, so
+// every load instruction reads one contiguous run of 64 x 16 bytes.  A chunk
+// with G < 64 lanes (an item's last) stores planes of G x 16 bytes.
+inline int stream_lane_entries(int dtype) { return dtype == CBSPMV_F64 ? 4 : 8; }
+inline int stream_val_bytes(int dtype) { return dtype == CBSPMV_F64 ? 8 : dtype == CBSPMV_F32 ? 4 : 2; }
 
-struct StreamSplit {       // one per strip cut into pieces
-    uint32_t strip;        // block-row id
+struct StreamSplit {       // one per row cut into pieces
+    uint32_t row;          // matrix row
     uint32_t first_item;   // item holding piece 0
-    uint32_t pieces;       // number of pieces (items) of this strip
-    uint32_t partial_base; // first slot in the partial buffer (slots of bs rows)
+    uint32_t pieces;       // number of pieces (items) of this row
+    uint32_t partial_base; // first slot in the partial buffer
 };
 
 struct Plan {
@@ -105,15 +127,16 @@ struct Plan {
     // Stream slab
     bool stream_ready = false;
     int col_bits = 0;
-    std::vector<uint32_t> s_key;
-    std::vector<double> s_val;       // FP64 copy; narrowed at upload
-    std::vector<int64_t> s_src;      // value source: input COO position of each entry (-1: summed/zero)
+    std::vector<uint32_t> s_key;     // logical entry order (export, tests)
+    std::vector<double> s_val;
+    std::vector<uint8_t> s_slab;     // device layout: chunk lane planes
     std::vector<StreamItem> items;
-    std::vector<uint32_t> strip_ids; // non-empty strips in order
-    std::vector<uint32_t> item_split;// per item: split id of its last strip or 0xffffffff
-    std::vector<uint32_t> item_split_first; // per item: split id of its first strip or 0xffffffff
+    std::vector<uint32_t> item_split;       // per item: split id of its last run or ~0
+    std::vector<uint32_t> item_split_first; // per item: split id of its first run or ~0
     std::vector<StreamSplit> splits;
-    std::vector<uint32_t> empty_strips;
+    std::vector<uint32_t> empty_rows;       // rows without entries (zeroed by the kernel)
+    std::vector<uint32_t> tiles;            // per tile: first line, line count
+    std::vector<uint32_t> tile_lines;       // cached x line ids (element / 16)
     uint32_t partial_slots = 0;
 };
 

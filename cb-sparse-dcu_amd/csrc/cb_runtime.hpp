@@ -24,8 +24,8 @@ struct DevicePlan {
     uint32_t n_zero_strips = 0;
     bool zero_all = false;   // LPT: every row is accumulated atomically
     // stream slab
-    DeviceBuf key, val, items, strip_ids, item_split, item_split_first, splits,
-        partial, counters, empty;
+    DeviceBuf key, val, items, item_split, item_split_first, splits,
+        partial, counters, empty, tiles, tile_lines;
     // transposed plan (backward)
     bool has_transpose = false;
     // SDDMM input coordinates

@@ -156,19 +156,24 @@ void pack_canonical(Plan& p) {
 }
 
 // ---- stream slab -------------------------------------------------------------
-// Every block of a strip (block row) contributes entries (column, row-in-strip,
-// value): COO blocks their merged entries, CSR blocks their entries, DENSE
-// blocks all bs*bs positions.  Columns are original columns: a gathered
+// Every block of a strip (block row) contributes entries (column, row, value):
+// COO blocks their merged entries, CSR blocks their entries, DENSE blocks
+// every position inside the matrix.  Columns are original columns: a gathered
 // block's local column goes through the restore map once, here, instead of
 // once per SpMV.  Inside a strip, entries are ordered by row, then block
-// order, then in-block order, so that the rows of a strip are contiguous runs
-// and a wave reduces them with a segmented scan instead of atomics.
+// order, then in-block order (a stable counting sort), so every row is one
+// contiguous run.
 void build_stream(Plan& p) {
     if (p.stream_ready) return;
     const int bs = p.o.bs, bcb = coord_bits(bs);
+    const int epl = stream_lane_entries(p.o.dtype);
+    const int slots = tile_slots(p.o.dtype), max_lines = tile_lines(p.o.dtype);
+    const uint32_t zero_slot = (uint32_t)slots - 1;
     int cb = 1;
-    while ((1ll << cb) < (long long)p.n) ++cb;
+    while ((1ll << cb) < (long long)std::max(p.n, slots)) ++cb;   // columns and LDS slots
     p.col_bits = cb;
+    const uint32_t cmask = (1u << cb) - 1u;
+    const uint32_t omask = (1u << (31 - cb)) - 1u;   // row-offset field
 
     // blocks of each strip in final block order (pad blocks skipped)
     std::vector<int> sb_ptr(p.gb_m + 1, 0), sb_list;
@@ -179,9 +184,8 @@ void build_stream(Plan& p) {
         std::vector<int> c(sb_ptr.begin(), sb_ptr.end() - 1);
         for (int g = 0; g < p.nnzb; ++g) if (p.nnzpb[g] > 0) sb_list[c[p.brow[g]]++] = g;
     }
-    // usable columns of a block: DENSE positions past the gathered column
-    // count (or past n) hold no entry and are skipped, as the reference's
-    // kernels skip them (cb-spmv-dcu.hpp:962, :1018, :1034)
+    // usable columns of a DENSE block: positions past the gathered column
+    // count (or past n) hold no entry (cb-spmv-dcu.hpp:962, :1018, :1034)
     auto block_cols_used = [&](int g) {
         const int b = p.brow[g];
         const int ncols = p.gather ? (p.block_cols_off[b + 1] - p.block_cols_off[b]) : p.n;
@@ -199,8 +203,8 @@ void build_stream(Plan& p) {
         strip_ptr[b + 1] = strip_ptr[b] + c;
     }
     const int64_t total = strip_ptr[p.gb_m];
-    p.s_key.assign(total, 0);
-    p.s_val.assign(total, 0.0);
+    std::vector<uint32_t> e_row(total), e_col(total);
+    std::vector<double> e_val(total);
 
     const uint32_t rmask = (1u << bcb) - 1;
 #pragma omp parallel
@@ -244,95 +248,196 @@ void build_stream(Plan& p) {
                         for (int c = 0; c < used; ++c) { tr.push_back(r); tk.push_back((uint32_t)orig_col(c)); tv.push_back(d[r * bs + c]); }
                 }
             }
-            // stable counting sort by row-in-strip
             std::fill(row_cnt.begin(), row_cnt.end(), 0);
             for (int r : tr) row_cnt[r + 1]++;
             for (int r = 0; r < bs; ++r) row_cnt[r + 1] += row_cnt[r];
             const int64_t base = strip_ptr[b];
             for (size_t k = 0; k < tr.size(); ++k) {
                 const int64_t at = base + row_cnt[tr[k]]++;
-                p.s_key[at] = tk[k] | ((uint32_t)tr[k] << cb);
-                p.s_val[at] = tv[k];
+                e_row[at] = (uint32_t)(b * bs + tr[k]);
+                e_col[at] = tk[k];
+                e_val[at] = tv[k];
             }
-            if (!tr.empty()) p.s_key[base] |= 0x80000000u;   // strip-start flag
         }
     }
 
-    // ---- wave items: runs of whole strips of about `target` entries (at
-    // most kMaxItemStrips strips).  Only a strip longer than 4 * target is
-    // cut, into pieces of about `target`; those pieces meet through the
-    // partial buffer, which costs device-scope fences, so cuts stay rare.
-    int64_t target = p.o.item_nnz > 0 ? p.o.item_nnz : 2048;
+    // rows without entries (the kernel stores their zeros)
+    p.empty_rows.clear();
+    {
+        std::vector<uint8_t> has(p.m, 0);
+        for (int64_t e = 0; e < total; ++e) if (e_row[e] < (uint32_t)p.m) has[e_row[e]] = 1;
+        for (int r = 0; r < p.m; ++r) if (!has[r]) p.empty_rows.push_back((uint32_t)r);
+    }
+
+    // ---- items: row runs packed into items of about `target` entries
+    // default: about 2048 items in all (256 tiles), 1K..16K entries each;
+    // longer items stage each x line in LDS once for more entries
+    int64_t target = p.o.item_nnz > 0 ? p.o.item_nnz
+                                      : std::max<int64_t>(1024, std::min<int64_t>(16384, (int64_t)p.nnz / 2048));
     target = std::max<int64_t>(64, std::min<int64_t>(target, 1 << 22));
     const int64_t split_len = 4 * target;
-    p.strip_ids.clear();
-    p.empty_strips.clear();
-    std::vector<int64_t> nz_ptr;
-    for (int b = 0; b < p.gb_m; ++b) {
-        if (strip_ptr[b + 1] > strip_ptr[b]) { p.strip_ids.push_back((uint32_t)b); nz_ptr.push_back(strip_ptr[b]); }
-        else p.empty_strips.push_back((uint32_t)b);
-    }
-    const int64_t NS = (int64_t)p.strip_ids.size();
-    nz_ptr.push_back(total);
-    p.items.clear();
-    int64_t s = 0;
-    while (s < NS) {
-        const int64_t len = nz_ptr[s + 1] - nz_ptr[s];
-        if (len > split_len) {
-            const int64_t pieces = (len + target - 1) / target;
-            for (int64_t q = 0; q < pieces; ++q) {
-                StreamItem it{};
-                it.e_begin = (uint32_t)(nz_ptr[s] + len * q / pieces);
-                it.e_end = (uint32_t)(nz_ptr[s] + len * (q + 1) / pieces);
-                it.s_begin = (uint32_t)s;
-                it.n_strips = 1;
-                it.first_partial = q > 0;
-                it.last_partial = q + 1 < pieces;
-                p.items.push_back(it);
-            }
-            ++s;
-            continue;
-        }
-        StreamItem it{};
-        it.s_begin = (uint32_t)s;
-        it.e_begin = (uint32_t)nz_ptr[s];
-        int64_t sum = 0, s0 = s;
-        while (s < NS && s - s0 < kMaxItemStrips && sum < target && nz_ptr[s + 1] - nz_ptr[s] <= split_len) {
-            sum += nz_ptr[s + 1] - nz_ptr[s];
-            ++s;
-        }
-        it.e_end = (uint32_t)nz_ptr[s];
-        it.n_strips = (uint16_t)(s - s0);
-        it.first_partial = 0;
-        it.last_partial = 0;
-        p.items.push_back(it);
-    }
-    // split strips: item i opens one with its last strip, interior items hold
-    // only that strip, and the closing item holds it as its first strip
-    const size_t NI = p.items.size();
+    struct Piece { int64_t b, e; uint8_t fp, lp; uint32_t split_first, split_last; };
+    std::vector<Piece> pieces;
     p.splits.clear();
-    p.item_split.assign(NI, 0xffffffffu);
-    p.item_split_first.assign(NI, 0xffffffffu);
     p.partial_slots = 0;
-    for (size_t i = 0; i < NI; ++i) {
-        const StreamItem& it = p.items[i];
-        if (!it.last_partial) continue;
-        const uint32_t sidx = it.s_begin + it.n_strips - 1;
-        size_t j = i + 1;
-        while (p.items[j].last_partial && p.items[j].n_strips == 1) ++j;   // interior pieces
-        StreamSplit sp{};
-        sp.strip = p.strip_ids[sidx];
-        sp.first_item = (uint32_t)i;
-        sp.pieces = (uint32_t)(j - i + 1);
-        sp.partial_base = p.partial_slots;
-        p.partial_slots += sp.pieces;
-        const uint32_t id = (uint32_t)p.splits.size();
-        p.splits.push_back(sp);
-        p.item_split[i] = id;
-        for (size_t q = i + 1; q < j; ++q) { p.item_split_first[q] = id; p.item_split[q] = id; }
-        p.item_split_first[j] = id;
-        i = j - 1;   // item j may open the next split with its own last strip
+    {
+        int64_t cur = 0;
+        int64_t r = 0;
+        while (r < total) {
+            int64_t re = r + 1;
+            while (re < total && e_row[re] == e_row[r]) ++re;
+            const int64_t len = re - r;
+            if (len > split_len) {
+                if (cur < r) pieces.push_back({cur, r, 0, 0, ~0u, ~0u});
+                const int64_t np = (len + target - 1) / target;
+                StreamSplit sp{};
+                sp.row = e_row[r];
+                sp.first_item = (uint32_t)pieces.size();
+                sp.pieces = (uint32_t)np;
+                sp.partial_base = p.partial_slots;
+                p.partial_slots += (uint32_t)np;
+                const uint32_t id = (uint32_t)p.splits.size();
+                p.splits.push_back(sp);
+                for (int64_t q = 0; q < np; ++q)
+                    pieces.push_back({r + len * q / np, r + len * (q + 1) / np, (uint8_t)(q > 0), (uint8_t)(q + 1 < np),
+                                      q > 0 ? id : ~0u, q + 1 < np ? id : ~0u});
+                cur = re;
+            } else if (r > cur && ((r - cur) + len > target || e_row[re - 1] - e_row[cur] > omask)) {
+                pieces.push_back({cur, r, 0, 0, ~0u, ~0u});
+                cur = r;
+            }
+            r = re;
+        }
+        if (cur < total) pieces.push_back({cur, total, 0, 0, ~0u, ~0u});
     }
+    // ---- padded entry stream and item table
+    const size_t NI = pieces.size();
+    p.items.assign(NI, StreamItem{});
+    p.item_split.assign(NI, ~0u);
+    p.item_split_first.assign(NI, ~0u);
+    std::vector<int64_t> out_at(NI + 1, 0);
+    for (size_t i = 0; i < NI; ++i) {
+        const int64_t len = pieces[i].e - pieces[i].b;
+        out_at[i + 1] = out_at[i] + (len + epl - 1) / epl * epl;
+    }
+    const int64_t padded = out_at[NI];
+    std::vector<uint32_t> p_col(padded), p_off(padded);
+    std::vector<uint8_t> p_zero(padded, 0);
+    p.s_val.assign(padded, 0.0);
+    for (size_t i = 0; i < NI; ++i) {
+        const Piece& pc = pieces[i];
+        StreamItem& it = p.items[i];
+        it.e_begin = (uint32_t)out_at[i];
+        it.e_end = (uint32_t)out_at[i + 1];
+        it.row_base = e_row[pc.b];
+        it.first_partial = pc.fp;
+        it.last_partial = pc.lp;
+        p.item_split_first[i] = pc.split_first;
+        p.item_split[i] = pc.split_last;
+        int64_t o = out_at[i];
+        for (int64_t e = pc.b; e < pc.e; ++e, ++o) {
+            p_col[o] = e_col[e];
+            p_off[o] = e_row[e] - it.row_base;
+            p.s_val[o] = e_val[e];
+        }
+        for (; o < out_at[i + 1]; ++o) {   // padding: extends the last run by 0 * 0
+            p_col[o] = 0;
+            p_off[o] = e_row[pc.e - 1] - it.row_base;
+            p_zero[o] = 1;
+        }
+    }
+    std::vector<uint32_t>().swap(e_row);
+    std::vector<uint32_t>().swap(e_col);
+    std::vector<double>().swap(e_val);
+
+    // ---- tiles: kItemsPerTile consecutive items; hot x lines staged in LDS
+    p.s_key.assign(padded, 0);
+    const size_t NT = (NI + kItemsPerTile - 1) / kItemsPerTile;
+    std::vector<std::vector<uint32_t>> lines(NT);
+#pragma omp parallel
+    {
+        std::vector<uint32_t> ln;
+        std::vector<std::pair<uint32_t, uint32_t>> cnt;   // (count, line)
+#pragma omp for schedule(dynamic, 4)
+        for (long long t = 0; t < (long long)NT; ++t) {
+            const int64_t b = p.items[t * kItemsPerTile].e_begin;
+            const int64_t e = p.items[std::min(NI, (size_t)(t + 1) * kItemsPerTile) - 1].e_end;
+            ln.clear();
+            for (int64_t q = b; q < e; ++q) if (!p_zero[q]) ln.push_back(p_col[q] / kLineElems);
+            std::sort(ln.begin(), ln.end());
+            cnt.clear();
+            for (size_t q = 0; q < ln.size();) {
+                size_t r = q + 1;
+                while (r < ln.size() && ln[r] == ln[q]) ++r;
+                if (r - q >= 2) cnt.push_back({(uint32_t)(r - q), ln[q]});
+                q = r;
+            }
+            if ((int)cnt.size() > max_lines) {
+                std::nth_element(cnt.begin(), cnt.begin() + max_lines, cnt.end(),
+                                 [](const std::pair<uint32_t, uint32_t>& x, const std::pair<uint32_t, uint32_t>& y) {
+                                     return x.first != y.first ? x.first > y.first : x.second < y.second;
+                                 });
+                cnt.resize(max_lines);
+            }
+            std::vector<uint32_t>& mine = lines[t];
+            mine.clear();
+            for (auto& c : cnt) mine.push_back(c.second);
+            std::sort(mine.begin(), mine.end());
+            for (int64_t q = b; q < e; ++q) {
+                const uint32_t off = (p_off[q] & omask) << cb;
+                if (p_zero[q]) { p.s_key[q] = kKeyCached | off | zero_slot; continue; }
+                const uint32_t col = p_col[q];
+                auto it = std::lower_bound(mine.begin(), mine.end(), col / kLineElems);
+                if (it != mine.end() && *it == col / kLineElems)
+                    p.s_key[q] = kKeyCached | off | ((uint32_t)(it - mine.begin()) * kLineElems + col % kLineElems);
+                else
+                    p.s_key[q] = off | col;
+            }
+        }
+    }
+    p.tiles.assign(2 * NT, 0);
+    p.tile_lines.clear();
+    for (size_t t = 0; t < NT; ++t) {
+        p.tiles[2 * t] = (uint32_t)p.tile_lines.size();
+        p.tiles[2 * t + 1] = (uint32_t)lines[t].size();
+        p.tile_lines.insert(p.tile_lines.end(), lines[t].begin(), lines[t].end());
+    }
+    (void)cmask;
+
+    // ---- device slab: per chunk, key planes then value planes (16 B/lane)
+    const int vb = stream_val_bytes(p.o.dtype);
+    const int nk = epl * 4 / 16, nv = epl * vb / 16;
+    const int64_t eb = 4 + vb;
+    p.s_slab.assign((size_t)(padded * eb), 0);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long long i = 0; i < (long long)NI; ++i) {
+        const StreamItem& it = p.items[i];
+        for (int64_t e0 = it.e_begin; e0 < it.e_end; e0 += 64 * epl) {
+            const int G = (int)std::min<int64_t>(64, (it.e_end - e0) / epl);
+            uint8_t* base = p.s_slab.data() + e0 * eb;
+            for (int l = 0; l < G; ++l)
+                for (int j = 0; j < epl; ++j) {
+                    const int64_t e = e0 + (int64_t)l * epl + j;
+                    std::memcpy(base + (size_t)(j / 4) * G * 16 + l * 16 + (j % 4) * 4, &p.s_key[e], 4);
+                    uint8_t* vdst = base + (size_t)(nk + j * vb / 16) * G * 16 + l * 16 + (j * vb) % 16;
+                    if (vb == 8) {
+                        std::memcpy(vdst, &p.s_val[e], 8);
+                    } else if (vb == 4) {
+                        const float f = (float)p.s_val[e];
+                        std::memcpy(vdst, &f, 4);
+                    } else {   // BF16, round to nearest even
+                        const float f = (float)p.s_val[e];
+                        uint32_t u;
+                        std::memcpy(&u, &f, 4);
+                        uint16_t h;
+                        if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) h = (uint16_t)((u >> 16) | 0x40);
+                        else h = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+                        std::memcpy(vdst, &h, 2);
+                    }
+                }
+        }
+    }
+    (void)nv;
     p.stream_ready = true;
 }
 
@@ -371,14 +476,16 @@ long export_array(const Plan& p, const std::string& s, void* dst, size_t cap, in
     if (s == "bia_end") return copy_out(p.bia_end, dst, cap);
     if (s == "image") return copy_out(p.image, dst, cap);
     if (s == "stream_key") return copy_out(p.s_key, dst, cap);
+    if (s == "stream_slab") return copy_out(p.s_slab, dst, cap);
     if (s == "stream_val") return copy_out(p.s_val, dst, cap);
-    if (s == "stream_strip_ids") return copy_out(p.strip_ids, dst, cap);
-    if (s == "stream_empty_strips") return copy_out(p.empty_strips, dst, cap);
+    if (s == "stream_empty_rows") return copy_out(p.empty_rows, dst, cap);
+    if (s == "stream_tiles") return copy_out(p.tiles, dst, cap);
+    if (s == "stream_tile_lines") return copy_out(p.tile_lines, dst, cap);
     if (s == "stream_items") {
         std::vector<uint32_t> v;
         for (const StreamItem& it : p.items) {
-            v.push_back(it.e_begin); v.push_back(it.e_end); v.push_back(it.s_begin);
-            v.push_back((uint32_t)it.n_strips | ((uint32_t)it.first_partial << 16) | ((uint32_t)it.last_partial << 24));
+            v.push_back(it.e_begin); v.push_back(it.e_end); v.push_back(it.row_base);
+            v.push_back((uint32_t)it.first_partial | ((uint32_t)it.last_partial << 8));
         }
         return copy_out(v, dst, cap);
     }

@@ -53,8 +53,9 @@ size_t val_elem_bytes(const Plan& h) { return h.o.dtype == CBSPMV_F64 ? 8 : h.o.
 }  // namespace
 
 void release_device(DevicePlan& d) {
-    for (DeviceBuf* b : {&d.slab, &d.zero_strips, &d.key, &d.val, &d.items, &d.strip_ids, &d.item_split,
-                         &d.item_split_first, &d.splits, &d.partial, &d.counters, &d.empty, &d.in_row, &d.in_col})
+    for (DeviceBuf* b : {&d.slab, &d.zero_strips, &d.key, &d.val, &d.items, &d.item_split,
+                         &d.item_split_first, &d.splits, &d.partial, &d.counters, &d.empty, &d.tiles, &d.tile_lines,
+                         &d.in_row, &d.in_col})
         release(*b);
     d.uploaded = false;
 }
@@ -71,10 +72,9 @@ int choose_schedule(const cbspmv_plan& p) {
 
 long long stream_bytes_per_spmv(const cbspmv_plan& p) {
     const Plan& h = p.host;
-    const long long E = (long long)h.s_key.size();
-    return E * (4 + (long long)val_elem_bytes(h)) + (long long)h.items.size() * (long long)sizeof(StreamItem) +
-           (long long)h.strip_ids.size() * 4 + (long long)h.m * (long long)vec_elem_bytes(h) +
-           (long long)h.n * (long long)vec_elem_bytes(h);
+    return (long long)h.s_slab.size() + (long long)h.items.size() * (long long)sizeof(StreamItem) +
+           (long long)h.tile_lines.size() * 4 + (long long)h.tiles.size() * 4 +
+           (long long)h.m * (long long)vec_elem_bytes(h) + (long long)h.n * (long long)vec_elem_bytes(h);
 }
 
 int upload_plan(cbspmv_plan& p, hipStream_t s) {
@@ -104,42 +104,24 @@ int upload_plan(cbspmv_plan& p, hipStream_t s) {
         if ((rc = upload_vec(d.zero_strips, zs, s))) return rc;
     }
     // stream slab
-    if ((rc = upload_vec(d.key, h.s_key, s))) return rc;
-    if (h.o.dtype == CBSPMV_F64) {
-        if ((rc = upload_vec(d.val, h.s_val, s))) return rc;
-    } else if (h.o.dtype == CBSPMV_F32) {
-        std::vector<float> v(h.s_val.begin(), h.s_val.end());
-        if ((rc = upload_vec(d.val, v, s))) return rc;
-    } else {
-        std::vector<uint16_t> v(h.s_val.size());
-        for (size_t i = 0; i < v.size(); ++i) {   // round to nearest even
-            const float f = (float)h.s_val[i];
-            uint32_t u;
-            std::memcpy(&u, &f, 4);
-            if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) { v[i] = (uint16_t)((u >> 16) | 0x40); continue; }
-            u += 0x7fffu + ((u >> 16) & 1u);
-            v[i] = (uint16_t)(u >> 16);
-        }
-        if ((rc = upload_vec(d.val, v, s))) return rc;
-    }
+    if ((rc = upload_vec(d.key, h.s_slab, s))) return rc;
     if ((rc = upload_vec(d.items, h.items, s))) return rc;
-    if ((rc = upload_vec(d.strip_ids, h.strip_ids, s))) return rc;
     if ((rc = upload_vec(d.item_split, h.item_split, s))) return rc;
     if ((rc = upload_vec(d.item_split_first, h.item_split_first, s))) return rc;
     if ((rc = upload_vec(d.splits, h.splits, s))) return rc;
-    if ((rc = upload_bytes(d.partial, nullptr, (size_t)h.partial_slots * h.o.bs * vec_elem_bytes(h), s))) return rc;
+    if ((rc = upload_bytes(d.partial, nullptr, (size_t)h.partial_slots * vec_elem_bytes(h), s))) return rc;
     if ((rc = upload_bytes(d.counters, nullptr, h.splits.size() * 4, s))) return rc;
-    if ((rc = upload_vec(d.empty, h.empty_strips, s))) return rc;
+    if ((rc = upload_vec(d.empty, h.empty_rows, s))) return rc;
+    if ((rc = upload_vec(d.tiles, h.tiles, s))) return rc;
+    if ((rc = upload_vec(d.tile_lines, h.tile_lines, s))) return rc;
     CBS_HIP(hipStreamSynchronize(s));
     d.uploaded = true;
     return CBSPMV_OK;
 }
 
 int stream_grid(const Plan& h) {
-    const long long waves = (long long)h.items.size();
-    long long grid = (waves + 3) / 4;
-    grid = std::max<long long>(8, (grid + 7) / 8 * 8);
-    return (int)std::min<long long>(grid, 1 << 20);
+    const long long tiles = (long long)h.tiles.size() / 2;
+    return (int)std::max<long long>(8, (tiles + 7) / 8 * 8);
 }
 
 int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s) {
@@ -148,43 +130,43 @@ int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s) {
     const int grid = stream_grid(h);
     hipError_t e;
     auto fill = [&](auto& a) {
-        a.key = static_cast<const uint32_t*>(d.key.ptr);
+        a.slab = static_cast<const uint8_t*>(d.key.ptr);
         a.items = static_cast<const StreamItem*>(d.items.ptr);
-        a.strip_ids = static_cast<const uint32_t*>(d.strip_ids.ptr);
         a.item_split = static_cast<const uint32_t*>(d.item_split.ptr);
         a.item_split_first = static_cast<const uint32_t*>(d.item_split_first.ptr);
         a.splits = static_cast<const StreamSplit*>(d.splits.ptr);
         a.counters = static_cast<uint32_t*>(d.counters.ptr);
-        a.empty = static_cast<const uint32_t*>(d.empty.ptr);
-        a.n_empty = (uint32_t)h.empty_strips.size();
+        a.tiles = static_cast<const uint32_t*>(d.tiles.ptr);
+        a.tile_lines = static_cast<const uint32_t*>(d.tile_lines.ptr);
+        a.empty_rows = static_cast<const uint32_t*>(d.empty.ptr);
+        a.n_empty_rows = (uint32_t)h.empty_rows.size();
         a.n_items = (uint32_t)h.items.size();
+        a.n_tiles = (uint32_t)(h.tiles.size() / 2);
         a.col_bits = h.col_bits;
         a.m = h.m;
+        a.n = h.n;
     };
     if (h.o.dtype == CBSPMV_F64) {
         StreamArgs<double, double> a{};
         fill(a);
-        a.val = static_cast<const double*>(d.val.ptr);
         a.partial = static_cast<double*>(d.partial.ptr);
         a.x = static_cast<const double*>(x);
         a.y = static_cast<double*>(y);
-        e = launch_stream(h.o.bs, a, grid, s);
+        e = launch_stream(a, grid, s);
     } else if (h.o.dtype == CBSPMV_F32) {
         StreamArgs<float, float> a{};
         fill(a);
-        a.val = static_cast<const float*>(d.val.ptr);
         a.partial = static_cast<float*>(d.partial.ptr);
         a.x = static_cast<const float*>(x);
         a.y = static_cast<float*>(y);
-        e = launch_stream(h.o.bs, a, grid, h.o.accum == CBSPMV_ACC_COMPENSATED, s);
+        e = launch_stream(a, grid, h.o.accum == CBSPMV_ACC_COMPENSATED, s);
     } else {
         StreamArgs<dev::bf16_t, float> a{};
         fill(a);
-        a.val = static_cast<const dev::bf16_t*>(d.val.ptr);
         a.partial = static_cast<float*>(d.partial.ptr);
         a.x = static_cast<const float*>(x);
         a.y = static_cast<float*>(y);
-        e = launch_stream(h.o.bs, a, grid, h.o.accum == CBSPMV_ACC_COMPENSATED, s);
+        e = launch_stream(a, grid, h.o.accum == CBSPMV_ACC_COMPENSATED, s);
     }
     if (e != hipSuccess) return hip_fail(e, "stream schedule launch");
     return CBSPMV_OK;

@@ -1,16 +1,20 @@
 // Stream schedule: y = A x over the transcoded strip stream (cb_plan.hpp).
 //
-// Each wave takes one item (about item_nnz entries, at most kMaxItemStrips
-// strips) at a time.  Keys and values stream in with non-temporal loads, U
-// 64-entry chunks in flight per lane; x is gathered through L2/MALL.  Because
-// a strip's entries are ordered by row, (strip-in-item, row) is non-decreasing
-// along the item, so each 64-lane chunk is reduced with one segmented scan and
-// each run's sum is added to a per-wave LDS accumulator [strip][row] by a
-// single lane (no atomics).  Whole strips are stored to y directly; a strip cut
-// into several items leaves partial rows in a small buffer and the last item
-// to finish it (an atomic ticket per split strip) sums the pieces in piece
-// order, so results are deterministic run to run.  Strips without entries are
-// zeroed by the same launch.
+// One workgroup (kItemsPerTile waves) per tile.  The workgroup first copies
+// the tile's hot x cache lines into LDS with coalesced loads, so entries whose
+// column is staged read LDS instead of issuing a divergent gather to L2 (one
+// L2 request per lane), which is what bounds a gather-heavy SpMV here.  Then
+// each wave streams its item: key and value loads for chunk c+1 are issued
+// (non-temporal) before chunk c is reduced, so every wave keeps a chunk of
+// HBM reads in flight.  Rows are contiguous runs inside an item, so a chunk is
+// reduced with one DPP segmented scan, each finished row is stored to y by its
+// last lane, and the row still open at the end of a chunk is carried into
+// the next one.  Items are cut at row boundaries, so rows have one writer and
+// need no atomics; only a row longer than four items is cut, and its pieces
+// meet in a partial buffer where the last piece to arrive (atomic ticket) sums
+// them in piece order.  Results are the same bits on every launch.
+#include <type_traits>
+
 #include "kernels.hpp"
 #include "stream.hpp"
 
@@ -22,122 +26,245 @@ __device__ __forceinline__ A load_partial(const A* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int BS, typename V, typename A, bool COMP>
+// (hi, lo) pairs for the compensated accumulation
+__device__ __forceinline__ f2& operator+=(f2& a, const f2& b) {
+    a = add_f2(a, b);
+    return a;
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ f2 dpp_f2(f2 v) { return f2{dpp<CTRL, ROWMASK>(v.hi), dpp<CTRL, ROWMASK>(v.lo)}; }
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void seg_step2(f2& v, int seg) {
+    const f2 pv = dpp_f2<CTRL, ROWMASK>(v);
+    const int ps = (int)dpp32<CTRL, ROWMASK>((uint32_t)seg, 0xffffffffu);
+    if (ps == seg) v += pv;
+}
+__device__ __forceinline__ f2 seg_scan_dpp(f2 v, int seg) {
+    seg_step2<kRowShr1, 0xf>(v, seg);
+    seg_step2<kRowShr2, 0xf>(v, seg);
+    seg_step2<kRowShr4, 0xf>(v, seg);
+    seg_step2<kRowShr8, 0xf>(v, seg);
+    seg_step2<kRowBcast15, 0xa>(v, seg);
+    seg_step2<kRowBcast31, 0xc>(v, seg);
+    return v;
+}
+
+// value of lane `src` (wave-uniform src) with v_readlane
+__device__ __forceinline__ float read_lane(float v, int src) {
+    return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v), src));
+}
+__device__ __forceinline__ double read_lane(double v, int src) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), src);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ f2 read_lane(f2 v, int src) { return f2{read_lane(v.hi, src), read_lane(v.lo, src)}; }
+
+template <typename A>
+__device__ __forceinline__ A finish(A v) { return v; }
+__device__ __forceinline__ float finish(f2 v) { return v.hi + v.lo; }
+
+// Chunk loads: 16-byte lane planes (cb_plan.hpp), non-temporal because the
+// slab is read once per SpMV and should not displace x from L2.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename V, int EPL>
+struct Chunk {
+    static constexpr int NK = EPL * 4 / 16;             // key planes
+    static constexpr int NV = EPL * (int)sizeof(V) / 16; // value planes
+    uint32_t k[EPL];
+    V v[EPL];
+    // chunk starting at entry e0 with G lanes; lanes >= G re-read lane G-1
+    __device__ __forceinline__ void load(const uint8_t* slab, uint32_t e0, uint32_t G, int lane) {
+        const uint8_t* base = slab + (size_t)e0 * (4 + sizeof(V));
+        const uint32_t l = min((uint32_t)lane, G - 1) * 16;
+#pragma unroll
+        for (int p = 0; p < NK; ++p) {
+            const u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(base + p * G * 16 + l));
+            k[4 * p] = w.x; k[4 * p + 1] = w.y; k[4 * p + 2] = w.z; k[4 * p + 3] = w.w;
+        }
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(base + (NK + q) * G * 16 + l));
+            uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+            __builtin_memcpy(&v[q * (16 / sizeof(V))], ws, 16);
+        }
+    }
+};
+
+// DPP whole-wave shifts (GFX9): wave_shl:1 (lane i reads i+1), wave_shr:1
+// (lane i reads i-1); the lane without a source reads `old`.
+template <int CTRL>
+__device__ __forceinline__ int wave_shift(int v, int old) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, 0xf, 0xf, false);
+}
+constexpr int kWaveShl1 = 0x130, kWaveShr1 = 0x138;
+__device__ __forceinline__ double wave_shr1(double v) { return dpp<kWaveShr1, 0xf>(v); }
+__device__ __forceinline__ float wave_shr1(float v) { return dpp<kWaveShr1, 0xf>(v); }
+__device__ __forceinline__ f2 wave_shr1(f2 v) { return f2{dpp<kWaveShr1, 0xf>(v.hi), dpp<kWaveShr1, 0xf>(v.lo)}; }
+
+template <typename V, typename A, bool COMP, int EPL>
 __global__ __launch_bounds__(kStreamThreads) void stream_spmv(StreamArgs<V, A> a) {
-    constexpr int WPB = kStreamThreads / kWave;
-    constexpr int U = sizeof(V) >= 8 ? 4 : 8;          // 64-entry chunks in flight per lane
-    constexpr int SLOTS = kMaxItemStrips * BS;
     using Acc = typename std::conditional<COMP, f2, A>::type;
-    __shared__ Acc acc_all[WPB][SLOTS];
+    constexpr uint32_t SLOTS = kTileBytes / sizeof(A);
+    constexpr uint32_t kZero = SLOTS - 1;
+    __shared__ A xs[SLOTS];
 
     const int lane = lane_id();
     const int wave = (int)(threadIdx.x >> 6);
-    // XCD-aware order: consecutive logical blocks (hence consecutive items and
-    // strips) run on the same XCD, so x windows stay in that XCD's L2.
-    uint32_t blk = blockIdx.x;
-    if ((gridDim.x & 7u) == 0) blk = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-    const uint32_t gwave = blk * WPB + wave;
-    const uint32_t nwaves = gridDim.x * WPB;
+    // XCD-aware order: the hardware deals workgroups round-robin over the 8
+    // XCDs, so map them to contiguous tile ranges per XCD (x windows of
+    // neighbouring tiles then share that XCD's L2).
+    uint32_t tile = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) tile = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
 
-    for (uint32_t t = gwave * kWave + lane; t < a.n_empty * BS; t += nwaves * kWave) {
-        const int64_t row = (int64_t)a.empty[t / BS] * BS + (t % BS);
-        if (row < a.m) a.y[row] = A(0);
+    for (uint32_t t = blockIdx.x * kStreamThreads + threadIdx.x; t < a.n_empty_rows; t += gridDim.x * kStreamThreads)
+        a.y[a.empty_rows[t]] = A(0);
+    if (tile >= a.n_tiles) return;
+
+    // ---- stage this tile's hot x lines in LDS: all line ids, then all x
+    // loads, issue back to back (16 consecutive threads read one line); the
+    // last slot stays 0 for the padding entries
+    {
+        constexpr int PER = SLOTS / kStreamThreads;
+        const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1] * kLineElems;
+        uint32_t lid[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t s = threadIdx.x + j * kStreamThreads;
+            lid[j] = s < cnt ? a.tile_lines[lb + s / kLineElems] : 0u;
+        }
+        A xv[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t s = threadIdx.x + j * kStreamThreads;
+            const uint32_t c = lid[j] * kLineElems + (s % kLineElems);
+            xv[j] = (s < cnt && c < (uint32_t)a.n) ? a.x[c] : A(0);
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t s = threadIdx.x + j * kStreamThreads;
+            if (s < cnt || s == kZero) xs[s] = xv[j];
+        }
     }
+    __syncthreads();
 
-    Acc* acc = acc_all[wave];
-    const uint32_t cmask = (1u << a.col_bits) - 1u;
-    for (uint32_t item = gwave; item < a.n_items; item += nwaves) {
-        const StreamItem it = a.items[item];
-        const int ns = it.n_strips;
-        for (int t = lane; t < ns * BS; t += kWave) {
-            if constexpr (COMP) acc[t] = f2{0.f, 0.f};
-            else acc[t] = A(0);
+    const uint32_t item = tile * kItemsPerTile + wave;
+    if (item >= a.n_items) return;
+    const StreamItem it = a.items[item];
+    const int cb = a.col_bits;
+    const uint32_t cmask = (1u << cb) - 1u;
+    const uint32_t omask = (1u << (31 - cb)) - 1u;
+    const uint32_t split_first = it.first_partial ? a.item_split_first[item] : 0xffffffffu;
+    const uint32_t split_last = it.last_partial ? a.item_split[item] : 0xffffffffu;
+
+    // A finished run (row offset `off`) goes to y, or to the partial buffer
+    // when it is a piece of a cut row (the item's first run, offset 0, or its
+    // last run).
+    auto emit = [&](bool go, int off, A v, bool final_run) {
+        if (!go) return;
+        if (it.first_partial && off == 0) {
+            const StreamSplit sp = a.splits[split_first];
+            a.partial[sp.partial_base + (item - sp.first_item)] = v;
+        } else if (final_run && it.last_partial) {
+            const StreamSplit sp = a.splits[split_last];
+            a.partial[sp.partial_base + (item - sp.first_item)] = v;
+        } else {
+            a.y[it.row_base + (uint32_t)off] = v;
         }
-        __builtin_amdgcn_wave_barrier();
-        int kbase = it.first_partial ? 0 : -1;
-        const uint32_t e_last = it.e_end - 1;
-        for (uint32_t e0 = it.e_begin; e0 < it.e_end; e0 += kWave * U) {
-            // all loads unconditional (index clamped to the item) so that
-            // the U key/value loads and then the U x gathers issue together
-            uint32_t key[U];
-            V val[U];
+    };
+    auto gather = [&](const uint32_t* kk, A* xx) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t e = min(e0 + u * kWave + lane, e_last);
-                key[u] = stream_load(a.key + e);
-                val[u] = stream_load(a.val + e);
-            }
-            A xv[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = a.x[key[u] & cmask];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const bool valid = e0 + u * kWave + lane <= e_last;
-                const uint64_t starts = __ballot(valid && (key[u] >> 31));
-                const int k = kbase + (int)__popcll(starts & lanes_le(lane));
-                kbase += (int)__popcll(starts);
-                const int r = (int)((key[u] >> a.col_bits) & (BS - 1));
-                const int seg = valid ? k * BS + r : 0x7fffffff;
-                Acc v;
-                if constexpr (COMP) {
-                    // segmented scan on (hi, lo) pairs with TwoSum
-                    v = valid ? two_prod(widen(val[u]), xv[u]) : f2{0.f, 0.f};
-#pragma unroll
-                    for (int d = 1; d < kWave; d <<= 1) {
-                        const float ph = shfl_up(v.hi, d), pl = shfl_up(v.lo, d);
-                        const int ps = shfl_up(seg, d);
-                        if (lane >= d && ps == seg) v = add_f2(v, f2{ph, pl});
-                    }
-                } else {
-                    v = seg_scan_dpp(valid ? A(widen(val[u])) * xv[u] : A(0), seg);
-                }
-                const int nseg = shfl_down(seg, 1);
-                const bool end = valid && (lane == kWave - 1 || nseg != seg);
-                if (end) {
-                    if constexpr (COMP) acc[seg] = add_f2(acc[seg], v);
-                    else atomicAdd(acc + seg, v);   // ds_add: run ends are unique per chunk
-                }
-            }
+        for (int j = 0; j < EPL; ++j) {
+            const uint32_t c = kk[j] & cmask;
+            const bool cached = (kk[j] & kKeyCached) != 0;
+            A g = A(0);
+            if (!cached) g = a.x[c];
+            const A l = xs[min(c, kZero)];
+            xx[j] = cached ? l : g;
         }
-        __builtin_amdgcn_wave_barrier();
-        // ---- write rows: whole strips to y, cut strips to the partial buffer
-        const uint32_t split_first = it.first_partial ? a.item_split_first[item] : 0xffffffffu;
-        const uint32_t split_last = it.last_partial ? a.item_split[item] : 0xffffffffu;
-        for (int t = lane; t < ns * BS; t += kWave) {
-            const int k = t / BS, r = t % BS;
-            A v;
-            if constexpr (COMP) v = acc[t].hi + acc[t].lo;
-            else v = acc[t];
-            const bool cut_first = k == 0 && it.first_partial;
-            const bool cut_last = k == ns - 1 && it.last_partial;
-            if (cut_first || cut_last) {
-                const StreamSplit sp = a.splits[cut_first ? split_first : split_last];
-                a.partial[(size_t)(sp.partial_base + (item - sp.first_item)) * BS + r] = v;
+    };
+    constexpr uint32_t kChunk = kWave * EPL;
+    auto lanes_of = [&](uint32_t e0) { return min(64u, (it.e_end - e0) / EPL); };
+
+    int carry_off = -1;
+    Acc carry{};
+    Chunk<V, EPL> cur, nxt;
+    A xc[EPL];
+    cur.load(a.slab, it.e_begin, lanes_of(it.e_begin), lane);
+    gather(cur.k, xc);
+    for (uint32_t e0 = it.e_begin; e0 < it.e_end; e0 += kChunk) {
+        const bool more = e0 + kChunk < it.e_end;
+        if (more) nxt.load(a.slab, e0 + kChunk, lanes_of(e0 + kChunk), lane);   // in flight while this chunk is reduced
+        const int L = (int)min(63u, (it.e_end - e0) / EPL - 1);   // last lane with entries
+        const bool valid = lane <= L;
+        int off[EPL];
+        Acc s[EPL];
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+            off[j] = valid ? (int)((cur.k[j] >> cb) & omask) : 0x7fffffff;
+            if constexpr (COMP) s[j] = two_prod(widen(cur.v[j]), xc[j]);
+            else s[j] = A(widen(cur.v[j])) * xc[j];
+        }
+        // the run left open by the previous chunk
+        if (carry_off >= 0) {
+            const int o0 = __builtin_amdgcn_readfirstlane(off[0]);
+            if (o0 == carry_off) {
+                if (lane == 0) s[0] += carry;
             } else {
-                const int64_t row = (int64_t)a.strip_ids[it.s_begin + k] * BS + r;
-                if (row < a.m) a.y[row] = v;
+                emit(lane == 0, carry_off, finish(carry), false);
             }
         }
-        // ---- last piece of a cut strip sums all pieces in order
-        for (int which = 0; which < 2; ++which) {
-            const uint32_t sid = which == 0 ? split_first : split_last;
-            if (sid == 0xffffffffu) continue;
-            if (which == 1 && sid == split_first) continue;   // interior piece: once
+        // runs inside the lane: s[j] becomes the partial sum of the run
+        // through entry j; ends[j] marks a run that stops at entry j
+        bool ends[EPL];
+#pragma unroll
+        for (int j = 1; j < EPL; ++j) {
+            const bool same = off[j] == off[j - 1];
+            ends[j - 1] = !same;
+            if (same) s[j] += s[j - 1];
+        }
+        // runs across lanes: segmented scan of the lanes' last runs
+        const Acc T = seg_scan_dpp(s[EPL - 1], off[EPL - 1]);
+        const int prev_off = wave_shift<kWaveShr1>(off[EPL - 1], -1);
+        const Acc prev_T = wave_shr1(T);
+        const int next_off = wave_shift<kWaveShl1>(off[0], -1);
+        bool first = true;   // still in the lane's first run
+#pragma unroll
+        for (int j = 0; j < EPL - 1; ++j) {
+            Acc v = s[j];
+            if (first && prev_off == off[0]) v += prev_T;
+            emit(valid && ends[j], off[j], finish(v), false);
+            first = first && !ends[j];
+        }
+        emit(valid && lane < L && next_off != off[EPL - 1], off[EPL - 1], finish(T), false);
+        carry = read_lane(T, L);
+        carry_off = __builtin_amdgcn_readlane(off[EPL - 1], L);
+        if (more) {
+            cur = nxt;
+            gather(cur.k, xc);
+        }
+    }
+    emit(lane == 0, carry_off, finish(carry), true);
+
+    // ---- pieces of a cut row: the last piece to arrive sums them in order
+    for (int which = 0; which < 2; ++which) {
+        const uint32_t sid = which == 0 ? split_first : split_last;
+        if (sid == 0xffffffffu || (which == 1 && sid == split_first)) continue;
+        __threadfence();
+        uint32_t ticket = 0;
+        if (lane == 0) ticket = atomicAdd(a.counters + sid, 1u);
+        ticket = __shfl(ticket, 0, kWave);
+        const StreamSplit sp = a.splits[sid];
+        if (ticket == sp.pieces - 1) {
             __threadfence();
-            uint32_t ticket = 0;
-            if (lane == 0) ticket = atomicAdd(a.counters + sid, 1u);
-            ticket = __shfl(ticket, 0, kWave);
-            const StreamSplit sp = a.splits[sid];
-            if (ticket == sp.pieces - 1) {
-                __threadfence();
-                for (int r = lane; r < BS; r += kWave) {
-                    A s = A(0);
-                    for (uint32_t q = 0; q < sp.pieces; ++q)
-                        s += load_partial(a.partial + (size_t)(sp.partial_base + q) * BS + r);
-                    const int64_t row = (int64_t)sp.strip * BS + r;
-                    if (row < a.m) a.y[row] = s;
-                }
-                if (lane == 0) atomicExch(a.counters + sid, 0u);
+            if (lane == 0) {
+                A sum = A(0);
+                for (uint32_t q = 0; q < sp.pieces; ++q) sum += load_partial(a.partial + sp.partial_base + q);
+                a.y[sp.row] = sum;
+                atomicExch(a.counters + sid, 0u);
             }
         }
     }
@@ -146,30 +273,21 @@ __global__ __launch_bounds__(kStreamThreads) void stream_spmv(StreamArgs<V, A> a
 }  // namespace dev
 
 template <typename V, typename A, bool COMP>
-hipError_t launch_stream_t(int bs, const StreamArgs<V, A>& a, int grid, hipStream_t s) {
-    const dim3 g(grid), b(kStreamThreads);
-    switch (bs) {
-        case 2: hipLaunchKernelGGL((dev::stream_spmv<2, V, A, COMP>), g, b, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((dev::stream_spmv<4, V, A, COMP>), g, b, 0, s, a); break;
-        case 8: hipLaunchKernelGGL((dev::stream_spmv<8, V, A, COMP>), g, b, 0, s, a); break;
-        case 16: hipLaunchKernelGGL((dev::stream_spmv<16, V, A, COMP>), g, b, 0, s, a); break;
-        case 32: hipLaunchKernelGGL((dev::stream_spmv<32, V, A, COMP>), g, b, 0, s, a); break;
-        case 64: hipLaunchKernelGGL((dev::stream_spmv<64, V, A, COMP>), g, b, 0, s, a); break;
-        default: return hipErrorInvalidValue;
-    }
+hipError_t launch_stream_t(const StreamArgs<V, A>& a, int grid, hipStream_t s) {
+    constexpr int EPL = sizeof(V) >= 8 ? 4 : 8;
+    hipLaunchKernelGGL((dev::stream_spmv<V, A, COMP, EPL>), dim3(grid), dim3(kStreamThreads), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_stream(int bs, const StreamArgs<double, double>& a, int grid, hipStream_t s) {
-    return launch_stream_t<double, double, false>(bs, a, grid, s);
+hipError_t launch_stream(const StreamArgs<double, double>& a, int grid, hipStream_t s) {
+    return launch_stream_t<double, double, false>(a, grid, s);
 }
-hipError_t launch_stream(int bs, const StreamArgs<float, float>& a, int grid, bool comp, hipStream_t s) {
-    return comp ? launch_stream_t<float, float, true>(bs, a, grid, s)
-                : launch_stream_t<float, float, false>(bs, a, grid, s);
+hipError_t launch_stream(const StreamArgs<float, float>& a, int grid, bool comp, hipStream_t s) {
+    return comp ? launch_stream_t<float, float, true>(a, grid, s) : launch_stream_t<float, float, false>(a, grid, s);
 }
-hipError_t launch_stream(int bs, const StreamArgs<dev::bf16_t, float>& a, int grid, bool comp, hipStream_t s) {
-    return comp ? launch_stream_t<dev::bf16_t, float, true>(bs, a, grid, s)
-                : launch_stream_t<dev::bf16_t, float, false>(bs, a, grid, s);
+hipError_t launch_stream(const StreamArgs<dev::bf16_t, float>& a, int grid, bool comp, hipStream_t s) {
+    return comp ? launch_stream_t<dev::bf16_t, float, true>(a, grid, s)
+                : launch_stream_t<dev::bf16_t, float, false>(a, grid, s);
 }
 
 }  // namespace cbs

@@ -10,30 +10,31 @@ namespace cbs {
 
 namespace dev { struct bf16_t; }
 
-constexpr int kStreamThreads = 256;   // 4 waves per workgroup
+constexpr int kStreamThreads = 64 * kItemsPerTile;   // one wave per tile item
 
 template <typename V, typename A>
 struct StreamArgs {
-    const uint32_t* key;
-    const V* val;
+    const uint8_t* slab;          // chunk lane planes (cb_plan.hpp)
     const StreamItem* items;
-    const uint32_t* strip_ids;
     const uint32_t* item_split;
     const uint32_t* item_split_first;
     const StreamSplit* splits;
+    const uint32_t* tiles;        // (first line, line count) per tile
+    const uint32_t* tile_lines;
+    const uint32_t* empty_rows;
     A* partial;
     uint32_t* counters;
-    const uint32_t* empty;
-    uint32_t n_empty;
+    uint32_t n_empty_rows;
     uint32_t n_items;
+    uint32_t n_tiles;
     int col_bits;
-    int m;
+    int m, n;
     const A* x;
     A* y;
 };
 
-hipError_t launch_stream(int bs, const StreamArgs<double, double>& a, int grid, hipStream_t s);
-hipError_t launch_stream(int bs, const StreamArgs<float, float>& a, int grid, bool comp, hipStream_t s);
-hipError_t launch_stream(int bs, const StreamArgs<dev::bf16_t, float>& a, int grid, bool comp, hipStream_t s);
+hipError_t launch_stream(const StreamArgs<double, double>& a, int grid, hipStream_t s);
+hipError_t launch_stream(const StreamArgs<float, float>& a, int grid, bool comp, hipStream_t s);
+hipError_t launch_stream(const StreamArgs<dev::bf16_t, float>& a, int grid, bool comp, hipStream_t s);
 
 }  // namespace cbs
