@@ -111,14 +111,25 @@ def main():
     ap.add_argument("--item-nnz", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device-to-device copy (for scaling PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU runs are out of scope", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
+
+    if args.calibrate_copy:
+        nbytes = 256 << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 5}), flush=True)
+        return
 
     import cbspmv
     from oracle import pyoracle

@@ -50,7 +50,9 @@ class Opts(ctypes.Structure):
         ("accumulate", ctypes.c_int),
         ("schedule", ctypes.c_int),
         ("stream_item_nnz", ctypes.c_int),
-        ("reserved", ctypes.c_int * 5),
+        ("stream_tile_waves", ctypes.c_int),
+        ("stream_tile_kib", ctypes.c_int),
+        ("reserved", ctypes.c_int * 3),
     ]
 
 

@@ -43,7 +43,7 @@ _SCHEDULES = {"auto": _lib.SCHED_AUTO, "blocked": _lib.SCHED_BLOCKED, "stream": 
 
 def make_opts(block_size=16, waves=4, row_owned=True, pattern_wave=True, force_coo64=False,
               dense_sixteenths=15, builder="gather", dtype="f64", compensated=False,
-              schedule="auto", item_nnz=0):
+              schedule="auto", item_nnz=0, tile_waves=0, tile_kib=0):
     o = _lib.Opts()
     lib().cbspmv_default_opts(ctypes.byref(o))
     o.block_size = int(block_size)
@@ -57,6 +57,8 @@ def make_opts(block_size=16, waves=4, row_owned=True, pattern_wave=True, force_c
     o.accumulate = _lib.ACC_COMPENSATED if compensated else _lib.ACC_NATIVE
     o.schedule = _SCHEDULES[schedule]
     o.stream_item_nnz = int(item_nnz)
+    o.stream_tile_waves = int(tile_waves)
+    o.stream_tile_kib = int(tile_kib)
     return o
 
 

@@ -75,6 +75,12 @@ int cbspmv_build(int m, int n, int nnz, const int* row, const int* col, const do
     h.o.accum = o.accumulate;
     h.o.schedule = o.schedule;
     h.o.item_nnz = o.stream_item_nnz;
+    if (o.stream_tile_waves) h.o.tile_items = o.stream_tile_waves;
+    if (o.stream_tile_kib) h.o.tile_bytes = o.stream_tile_kib * 1024;
+    if (h.o.tile_items < 1 || h.o.tile_items > cbs::kMaxTileItems || h.o.tile_bytes < 4096 || h.o.tile_bytes > 160 * 1024) {
+        delete p;
+        return fail(CBSPMV_ERR_ARG, "stream tile shape out of range (1..16 waves, 4..160 KiB)");
+    }
     std::string err;
     int rc;
     try {

@@ -16,6 +16,9 @@ namespace cbs {
 // Block type codes and flags (cb-spmv-dcu/src/macros.h:99-105).
 enum : uint8_t { kCOO = 0, kCSR = 1, kDENSE = 3, kTypeMask = 0x7f, kUniform = 0x80 };
 
+constexpr int kDefaultTileItemsOpt = 8;
+constexpr int kDefaultTileBytesOpt = 96 * 1024;
+
 struct Options {
     int bs = 16;
     int waves = 4;
@@ -28,6 +31,8 @@ struct Options {
     int accum = CBSPMV_ACC_NATIVE;
     int schedule = CBSPMV_SCHED_AUTO;
     int item_nnz = 0;
+    int tile_items = kDefaultTileItemsOpt;
+    int tile_bytes = kDefaultTileBytesOpt;
 };
 
 inline int coord_bits(int bs) { int b = 0; while ((1 << b) < bs) ++b; return b; }
@@ -48,7 +53,7 @@ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) & ~(a - 1); }
 // than 4 items is cut, and its pieces meet in a small partial buffer through
 // an atomic ticket.  Every item is padded with zero entries (value 0 times
 // the always-zero LDS slot) to a multiple of the entries each lane owns, so
-// lanes are either full or idle.  kItemsPerTile consecutive items form a tile
+// lanes are either full or idle.  tile_items consecutive items form a tile
 // run by one workgroup, which first stages the tile's hot x cache lines (16
 // elements each, lines hit at least twice) into LDS.
 struct StreamItem {        // 16 bytes
@@ -59,13 +64,15 @@ struct StreamItem {        // 16 bytes
     uint8_t last_partial;  // last run continues into the next item
     uint8_t pad[2];
 };
-constexpr int kItemsPerTile = 8;        // waves per stream workgroup
-constexpr int kTileBytes = 80 * 1024;   // LDS x cache per workgroup (2 per CU)
 constexpr int kLineElems = 16;          // x elements per cached line
 constexpr uint32_t kKeyCached = 0x80000000u;
+// Tile shape: waves (items) per workgroup and LDS bytes of x cache; the
+// defaults give one 8-wave workgroup per CU with 96 KB of its 160 KB of LDS.
+constexpr int kMaxTileItems = 16;
+constexpr int kStreamCUs = 256;         // MI355X compute units (default tile count)
 // LDS x-cache elements per workgroup; the last slot always holds 0
-inline int tile_slots(int dtype) { return kTileBytes / (dtype == CBSPMV_F64 ? 8 : 4); }
-inline int tile_lines(int dtype) { return tile_slots(dtype) / kLineElems - 1; }
+inline int tile_slots(int bytes, int dtype) { return bytes / (dtype == CBSPMV_F64 ? 8 : 4); }
+inline int tile_lines(int bytes, int dtype) { return tile_slots(bytes, dtype) / kLineElems - 1; }
 // Entries one lane owns per chunk (a chunk is 64 lanes' worth).  The slab
 // stores each chunk as 16-byte lane planes: the keys (1 or 2 planes), then
 // the values (1 or 2 planes); plane p holds 16 bytes per lane, lane-major, so

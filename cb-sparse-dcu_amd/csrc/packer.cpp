@@ -167,7 +167,8 @@ void build_stream(Plan& p) {
     if (p.stream_ready) return;
     const int bs = p.o.bs, bcb = coord_bits(bs);
     const int epl = stream_lane_entries(p.o.dtype);
-    const int slots = tile_slots(p.o.dtype), max_lines = tile_lines(p.o.dtype);
+    const int slots = tile_slots(p.o.tile_bytes, p.o.dtype), max_lines = tile_lines(p.o.tile_bytes, p.o.dtype);
+    const size_t tile_items = (size_t)p.o.tile_items;
     const uint32_t zero_slot = (uint32_t)slots - 1;
     int cb = 1;
     while ((1ll << cb) < (long long)std::max(p.n, slots)) ++cb;   // columns and LDS slots
@@ -269,18 +270,17 @@ void build_stream(Plan& p) {
         for (int r = 0; r < p.m; ++r) if (!has[r]) p.empty_rows.push_back((uint32_t)r);
     }
 
-    // ---- items: row runs packed into items of about `target` entries
-    // default: about 2048 items in all (256 tiles), 1K..16K entries each;
-    // longer items stage each x line in LDS once for more entries
-    int64_t target = p.o.item_nnz > 0 ? p.o.item_nnz
-                                      : std::max<int64_t>(1024, std::min<int64_t>(16384, (int64_t)p.nnz / 2048));
-    target = std::max<int64_t>(64, std::min<int64_t>(target, 1 << 22));
-    const int64_t split_len = 4 * target;
+    // ---- items: row runs packed into items of about `target` entries.  By
+    // default the target is chosen so that the tiles (tile_items items each)
+    // number at most kStreamCUs, i.e. every workgroup of a launch runs in one
+    // round on a 256-CU MI355X; an explicit item_nnz is used as given.
     struct Piece { int64_t b, e; uint8_t fp, lp; uint32_t split_first, split_last; };
     std::vector<Piece> pieces;
-    p.splits.clear();
-    p.partial_slots = 0;
-    {
+    auto cut = [&](int64_t target) {
+        const int64_t split_len = 4 * target;
+        pieces.clear();
+        p.splits.clear();
+        p.partial_slots = 0;
         int64_t cur = 0;
         int64_t r = 0;
         while (r < total) {
@@ -309,6 +309,17 @@ void build_stream(Plan& p) {
             r = re;
         }
         if (cur < total) pieces.push_back({cur, total, 0, 0, ~0u, ~0u});
+    };
+    if (p.o.item_nnz > 0) {
+        cut(std::max<int64_t>(64, std::min<int64_t>(p.o.item_nnz, 1 << 22)));
+    } else {
+        const int64_t max_items = (int64_t)kStreamCUs * p.o.tile_items;
+        int64_t target = std::max<int64_t>(1024, total / max_items + 1);
+        for (int round = 0; round < 64; ++round) {
+            cut(target);
+            if ((int64_t)pieces.size() <= max_items || target >= (1 << 22)) break;
+            target = target + target / 16 + 1;
+        }
     }
     // ---- padded entry stream and item table
     const size_t NI = pieces.size();
@@ -350,9 +361,9 @@ void build_stream(Plan& p) {
     std::vector<uint32_t>().swap(e_col);
     std::vector<double>().swap(e_val);
 
-    // ---- tiles: kItemsPerTile consecutive items; hot x lines staged in LDS
+    // ---- tiles: tile_items consecutive items; hot x lines staged in LDS
     p.s_key.assign(padded, 0);
-    const size_t NT = (NI + kItemsPerTile - 1) / kItemsPerTile;
+    const size_t NT = (NI + tile_items - 1) / tile_items;
     std::vector<std::vector<uint32_t>> lines(NT);
 #pragma omp parallel
     {
@@ -360,8 +371,8 @@ void build_stream(Plan& p) {
         std::vector<std::pair<uint32_t, uint32_t>> cnt;   // (count, line)
 #pragma omp for schedule(dynamic, 4)
         for (long long t = 0; t < (long long)NT; ++t) {
-            const int64_t b = p.items[t * kItemsPerTile].e_begin;
-            const int64_t e = p.items[std::min(NI, (size_t)(t + 1) * kItemsPerTile) - 1].e_end;
+            const int64_t b = p.items[t * tile_items].e_begin;
+            const int64_t e = p.items[std::min(NI, (size_t)(t + 1) * tile_items) - 1].e_end;
             ln.clear();
             for (int64_t q = b; q < e; ++q) if (!p_zero[q]) ln.push_back(p_col[q] / kLineElems);
             std::sort(ln.begin(), ln.end());
@@ -490,6 +501,7 @@ long export_array(const Plan& p, const std::string& s, void* dst, size_t cap, in
         return copy_out(v, dst, cap);
     }
     if (s == "stream_col_bits") return copy_out(std::vector<int>{p.col_bits}, dst, cap);
+    if (s == "stream_tile_shape") return copy_out(std::vector<int>{p.o.tile_items, p.o.tile_bytes}, dst, cap);
     if (s == "pl_idx" || s == "pl_val") {
         const int bs = p.o.bs;
         std::vector<int32_t> idx;

@@ -142,6 +142,8 @@ int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s) {
         a.n_empty_rows = (uint32_t)h.empty_rows.size();
         a.n_items = (uint32_t)h.items.size();
         a.n_tiles = (uint32_t)(h.tiles.size() / 2);
+        a.tile_items = (uint32_t)h.o.tile_items;
+        a.tile_slots = (uint32_t)tile_slots(h.o.tile_bytes, h.o.dtype);
         a.col_bits = h.col_bits;
         a.m = h.m;
         a.n = h.n;

@@ -1,6 +1,6 @@
 // Stream schedule: y = A x over the transcoded strip stream (cb_plan.hpp).
 //
-// One workgroup (kItemsPerTile waves) per tile.  The workgroup first copies
+// One workgroup (tile_items waves) per tile.  The workgroup first copies
 // the tile's hot x cache lines into LDS with coalesced loads, so entries whose
 // column is staged read LDS instead of issuing a divergent gather to L2 (one
 // L2 request per lane), which is what bounds a gather-heavy SpMV here.  Then
@@ -106,11 +106,13 @@ __device__ __forceinline__ float wave_shr1(float v) { return dpp<kWaveShr1, 0xf>
 __device__ __forceinline__ f2 wave_shr1(f2 v) { return f2{dpp<kWaveShr1, 0xf>(v.hi), dpp<kWaveShr1, 0xf>(v.lo)}; }
 
 template <typename V, typename A, bool COMP, int EPL>
-__global__ __launch_bounds__(kStreamThreads) void stream_spmv(StreamArgs<V, A> a) {
+__global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A> a) {
     using Acc = typename std::conditional<COMP, f2, A>::type;
-    constexpr uint32_t SLOTS = kTileBytes / sizeof(A);
-    constexpr uint32_t kZero = SLOTS - 1;
-    __shared__ A xs[SLOTS];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    A* xs = reinterpret_cast<A*>(lds_raw);   // tile_slots elements (dynamic LDS)
+    const uint32_t SLOTS = a.tile_slots;
+    const uint32_t kZero = SLOTS - 1;
+    const uint32_t nthreads = blockDim.x;
 
     const int lane = lane_id();
     const int wave = (int)(threadIdx.x >> 6);
@@ -120,38 +122,42 @@ __global__ __launch_bounds__(kStreamThreads) void stream_spmv(StreamArgs<V, A> a
     uint32_t tile = blockIdx.x;
     if ((gridDim.x & 7u) == 0) tile = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
 
-    for (uint32_t t = blockIdx.x * kStreamThreads + threadIdx.x; t < a.n_empty_rows; t += gridDim.x * kStreamThreads)
+    for (uint32_t t = blockIdx.x * nthreads + threadIdx.x; t < a.n_empty_rows; t += gridDim.x * nthreads)
         a.y[a.empty_rows[t]] = A(0);
     if (tile >= a.n_tiles) return;
 
-    // ---- stage this tile's hot x lines in LDS: all line ids, then all x
-    // loads, issue back to back (16 consecutive threads read one line); the
-    // last slot stays 0 for the padding entries
+    // ---- stage this tile's hot x lines in LDS, PER slots per thread per
+    // pass with all line-id loads, then all x loads, issued back to back (16
+    // consecutive threads read one line); the last slot stays 0 for the
+    // padding entries
     {
-        constexpr int PER = SLOTS / kStreamThreads;
+        constexpr int PER = 8;
         const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1] * kLineElems;
-        uint32_t lid[PER];
+        for (uint32_t s0 = threadIdx.x; s0 < cnt; s0 += PER * nthreads) {
+            uint32_t lid[PER];
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            const uint32_t s = threadIdx.x + j * kStreamThreads;
-            lid[j] = s < cnt ? a.tile_lines[lb + s / kLineElems] : 0u;
-        }
-        A xv[PER];
+            for (int j = 0; j < PER; ++j) {
+                const uint32_t s = s0 + j * nthreads;
+                lid[j] = s < cnt ? a.tile_lines[lb + s / kLineElems] : 0u;
+            }
+            A xv[PER];
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            const uint32_t s = threadIdx.x + j * kStreamThreads;
-            const uint32_t c = lid[j] * kLineElems + (s % kLineElems);
-            xv[j] = (s < cnt && c < (uint32_t)a.n) ? a.x[c] : A(0);
-        }
+            for (int j = 0; j < PER; ++j) {
+                const uint32_t s = s0 + j * nthreads;
+                const uint32_t c = lid[j] * kLineElems + (s % kLineElems);
+                xv[j] = (s < cnt && c < (uint32_t)a.n) ? a.x[c] : A(0);
+            }
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            const uint32_t s = threadIdx.x + j * kStreamThreads;
-            if (s < cnt || s == kZero) xs[s] = xv[j];
+            for (int j = 0; j < PER; ++j) {
+                const uint32_t s = s0 + j * nthreads;
+                if (s < cnt) xs[s] = xv[j];
+            }
         }
+        if (threadIdx.x == 0) xs[kZero] = A(0);
     }
     __syncthreads();
 
-    const uint32_t item = tile * kItemsPerTile + wave;
+    const uint32_t item = tile * a.tile_items + wave;
     if (item >= a.n_items) return;
     const StreamItem it = a.items[item];
     const int cb = a.col_bits;
@@ -275,7 +281,15 @@ __global__ __launch_bounds__(kStreamThreads) void stream_spmv(StreamArgs<V, A> a
 template <typename V, typename A, bool COMP>
 hipError_t launch_stream_t(const StreamArgs<V, A>& a, int grid, hipStream_t s) {
     constexpr int EPL = sizeof(V) >= 8 ? 4 : 8;
-    hipLaunchKernelGGL((dev::stream_spmv<V, A, COMP, EPL>), dim3(grid), dim3(kStreamThreads), 0, s, a);
+    auto kern = dev::stream_spmv<V, A, COMP, EPL>;
+    const size_t lds = (size_t)a.tile_slots * sizeof(A);
+    static bool attr_set = false;   // allow more than 64 KB of dynamic LDS
+    if (!attr_set) {
+        const hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * a.tile_items), lds, s, a);
     return hipGetLastError();
 }
 

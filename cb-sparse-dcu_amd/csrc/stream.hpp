@@ -10,7 +10,7 @@ namespace cbs {
 
 namespace dev { struct bf16_t; }
 
-constexpr int kStreamThreads = 64 * kItemsPerTile;   // one wave per tile item
+constexpr int kMaxStreamThreads = 64 * kMaxTileItems;   // one wave per tile item
 
 template <typename V, typename A>
 struct StreamArgs {
@@ -27,6 +27,8 @@ struct StreamArgs {
     uint32_t n_empty_rows;
     uint32_t n_items;
     uint32_t n_tiles;
+    uint32_t tile_items;          // waves per workgroup
+    uint32_t tile_slots;          // LDS x-cache elements
     int col_bits;
     int m, n;
     const A* x;

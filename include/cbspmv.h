@@ -69,7 +69,9 @@ typedef struct cbspmv_opts {
     int accumulate;       /* enum cbspmv_accum (F32/BF16 only)                       */
     int schedule;         /* enum cbspmv_schedule                                    */
     int stream_item_nnz;  /* stream schedule: target entries per wave item (0=auto)  */
-    int reserved[5];
+    int stream_tile_waves;/* stream schedule: waves per workgroup, 1..16 (0=auto: 8)   */
+    int stream_tile_kib;  /* stream schedule: LDS x cache per workgroup, KiB (0=96)  */
+    int reserved[3];
 } cbspmv_opts;
 
 /* Builder statistics: the GLOBAL_BLOCK scalars (format.h:63-77) plus counts. */

@@ -8,13 +8,14 @@ from cbspmv import Plan, gen
 from oracle import pyoracle
 
 CACHED = np.uint32(0x80000000)
-ITEMS_PER_TILE = 8
 LINE = 16
-TILE_BYTES = 80 * 1024
+def tile_shape(plan):
+    waves, nbytes = plan.export("stream_tile_shape")
+    return int(waves), int(nbytes)
 
 
-def tile_slots(dtype):
-    return TILE_BYTES // (8 if dtype == "f64" else 4)
+def tile_slots(plan, dtype):
+    return tile_shape(plan)[1] // (8 if dtype == "f64" else 4)
 
 
 def decode_stream(plan, dtype="f64"):
@@ -34,8 +35,8 @@ def decode_stream(plan, dtype="f64"):
     item_of = np.repeat(np.arange(items.shape[0]), items[:, 1] - items[:, 0])
     assert item_of.size == key.size
     row = items[item_of, 2] + off
-    tile_of = item_of // ITEMS_PER_TILE
-    pad = cached & (low == tile_slots(dtype) - 1)
+    tile_of = item_of // tile_shape(plan)[0]
+    pad = cached & (low == tile_slots(plan, dtype) - 1)
     col = low.copy()
     cidx = np.nonzero(cached & ~pad)[0]
     slot = low[cidx]
@@ -98,7 +99,7 @@ def test_stream_decodes_to_input_matrix(bs, item_nnz, dtype):
             same_row = row_all[items[i, 0]] == row_all[items[i - 1, 1] - 1]
             assert bool(same_row) == bool(first_partial[i]) == bool(last_partial[i - 1]), (name, i)
         # staged lines: sorted, distinct, at most 511 per tile
-        assert np.all(tiles[:, 1] <= tile_slots(dtype) // LINE - 1)
+        assert np.all(tiles[:, 1] <= tile_slots(plan, dtype) // LINE - 1)
         for t in range(tiles.shape[0]):
             ls = lines[tiles[t, 0]: tiles[t, 0] + tiles[t, 1]]
             assert np.all(np.diff(ls) > 0)
