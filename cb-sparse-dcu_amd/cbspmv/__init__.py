@@ -5,7 +5,7 @@ include/cbspmv.h).  This package binds it with ctypes and uses torch only for
 device memory and streams.
 """
 from ._lib import CbspmvError, LIB_PATH, lib  # noqa: F401
-from .plan import ARRAY_NAMES, Plan, make_opts  # noqa: F401
+from .plan import ARRAY_NAMES, Plan, make_opts, read_mtx  # noqa: F401
 from .reference_api import (  # noqa: F401
     GLOBAL_BLOCK,
     PROGRAM_METRICS,

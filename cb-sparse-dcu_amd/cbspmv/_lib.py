@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcbspmv.so")
 
 CBSPMV_OK = 0
+ERR_ARG, ERR_RANGE, ERR_ORDER, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_UNSUPPORTED = -1, -2, -3, -4, -5, -6, -7
 ERRORS = {
     -1: "CBSPMV_ERR_ARG",
     -2: "CBSPMV_ERR_RANGE",

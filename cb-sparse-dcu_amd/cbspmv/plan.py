@@ -24,6 +24,7 @@ _EXPORT_DTYPES = {
     "stream_key": np.uint32,
     "stream_val": np.float64,
     "stream_items": np.uint32,
+    "policy": np.int32,
     "stream_slab": np.uint8,
     "stream_tiles": np.uint32,
     "stream_tile_lines": np.uint32,
@@ -169,3 +170,30 @@ class Plan:
         check(lib().cbspmv_bench(self._h, _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double), ctypes.byref(met)))
         self._uploaded = True
         return y, met.runtime, met.gflops
+
+
+def read_mtx(path):
+    """Matrix Market file -> (m, n, row, col, val) with the reference
+    reader's semantics (values forced to 1, symmetric expansion, row-major
+    COO; cb-spmv-dcu/src/io/mmio_highlevel.h:62-403)."""
+    L = lib()
+    f = L.cbspmv_read_mtx
+    f.restype = ctypes.c_int
+    ip = ctypes.POINTER(ctypes.c_int)
+    f.argtypes = [ctypes.c_char_p, ip, ip, ip, ctypes.POINTER(ip), ctypes.POINTER(ip),
+                  ctypes.POINTER(ctypes.POINTER(ctypes.c_double))]
+    L.cbspmv_free.argtypes = [ctypes.c_void_p]
+    m, n, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    r, c, v = ip(), ip(), ctypes.POINTER(ctypes.c_double)()
+    check(f(str(path).encode(), ctypes.byref(m), ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(r),
+            ctypes.byref(c), ctypes.byref(v)))
+    try:
+        k = nnz.value
+        row = np.ctypeslib.as_array(r, shape=(k,)).copy() if k else np.zeros(0, np.int32)
+        col = np.ctypeslib.as_array(c, shape=(k,)).copy() if k else np.zeros(0, np.int32)
+        val = np.ctypeslib.as_array(v, shape=(k,)).copy() if k else np.zeros(0)
+    finally:
+        L.cbspmv_free(ctypes.cast(r, ctypes.c_void_p))
+        L.cbspmv_free(ctypes.cast(c, ctypes.c_void_p))
+        L.cbspmv_free(ctypes.cast(v, ctypes.c_void_p))
+    return m.value, n.value, row.astype(np.int32), col.astype(np.int32), val

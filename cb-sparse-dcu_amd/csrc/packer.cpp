@@ -501,6 +501,20 @@ long export_array(const Plan& p, const std::string& s, void* dst, size_t cap, in
         return copy_out(v, dst, cap);
     }
     if (s == "stream_col_bits") return copy_out(std::vector<int>{p.col_bits}, dst, cap);
+    if (s == "policy") {
+        // the reference's dispatch decision (cb-spmv-dcu.hpp:1428-1450):
+        // unit_values, use_pattern, use_unrolled, use_small_direct,
+        // launch_waves, grid, aggregate_coo, pattern_count
+        const long P = p.pat_row_ptr.empty() ? 0 : (long)p.pat_row_ptr.size() - 1;
+        const bool pat = p.o.pattern_wave && p.unit_values && P > 0 && 2 * P < p.m;
+        const bool unr = pat && (long)p.pat_col.size() >= P * 128;
+        const bool small = p.o.row_owned && !pat && p.unit_values && p.gather && p.compact && p.nnz < 10000;
+        const long waves = (pat && P < 1000) ? 9 : p.o.waves;
+        const long grid = pat ? (P + waves - 1) / waves
+                              : p.o.row_owned ? (long)p.seg_rows.size() : ((long)p.nnzb + p.o.waves - 1) / p.o.waves;
+        return copy_out(std::vector<int>{(int)p.unit_values, (int)pat, (int)unr, (int)small, (int)waves, (int)grid,
+                                         (int)(p.nnz >= 10000), (int)P}, dst, cap);
+    }
     if (s == "stream_tile_shape") return copy_out(std::vector<int>{p.o.tile_items, p.o.tile_bytes}, dst, cap);
     if (s == "pl_idx" || s == "pl_val") {
         const int bs = p.o.bs;

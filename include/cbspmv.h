@@ -145,43 +145,17 @@ int cbspmv_get_schedule(const cbspmv_plan* plan);
  * x, y are host arrays of n and m doubles.  Timings go to *metrics. */
 int cbspmv_bench(cbspmv_plan* plan, const double* x, double* y, cbspmv_metrics* metrics);
 
-/* ---- differentiation (no reference counterpart; extension) ------------------ */
-/* Builds the transposed CB structure once (A^T, same options). */
-int cbspmv_build_transpose(cbspmv_plan* plan);
-/* d_dx = A^T d_g  (n elements). */
-int cbspmv_spmv_transpose(cbspmv_plan* plan, const void* d_g, void* d_dx, void* stream);
-/* d_dvals[k] = g[row[k]] * x[col[k]] for every input COO entry k, in input
- * order (the blockwise SDDMM over the stored pattern).  Also computes
- * d_dx = A^T d_g when d_dx != NULL, in the same pass over the transposed plan. */
-int cbspmv_backward(cbspmv_plan* plan, const void* d_g, const void* d_x,
-                    void* d_dx, void* d_dvals, void* stream);
-/* Replaces the values of A (input COO order, plan's value dtype promoted to
- * the vector type: double for F64 plans, float otherwise) on the device. */
-int cbspmv_set_values(cbspmv_plan* plan, const void* d_vals, void* stream);
-
-/* ---- batching (no reference counterpart; extension) -------------------------- */
-typedef struct cbspmv_batch cbspmv_batch;
-/* Builds one batch from `count` plans already built (borrowed; they must
- * outlive the batch) and uploads a single descriptor table. */
-int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** batch, void* stream);
-/* Y_i = A_i X_i for every matrix i, with nrhs[i] right-hand sides each:
- * d_x[i] is n_i x nrhs[i] (column-major, leading dim n_i), d_y[i] is
- * m_i x nrhs[i].  One persistent launch. */
-int cbspmv_batch_spmv(cbspmv_batch* batch, const void* const* d_x, void* const* d_y,
-                      const int* nrhs, void* stream);
-/* Captures the launch above into a hipGraph bound to these pointers and
- * replays it; cbspmv_batch_replay() launches the captured graph again. */
-int cbspmv_batch_capture(cbspmv_batch* batch, const void* const* d_x, void* const* d_y,
-                         const int* nrhs, void* stream);
-int cbspmv_batch_replay(cbspmv_batch* batch, void* stream);
-void cbspmv_batch_destroy(cbspmv_batch* batch);
-
-/* Autotuner: builds the candidate plans (block size x waves) for one COO
- * matrix, times each on the device and returns the fastest, destroying the
- * others.  `candidates` lists (block_size, waves) pairs; NULL = {16,32}x{4,9}. */
-int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const double* val,
-                    const cbspmv_opts* base, const int* candidates, int ncandidates,
-                    int nrhs, cbspmv_plan** best, double* best_ms, void* stream);
+/* ---- input -------------------------------------------------------------------- */
+/* Reads a Matrix Market coordinate file the way the reference's reader does
+ * (cb-spmv-dcu/src/io/mmio_highlevel.h:62-403, read_matrix_coo_data): every
+ * value is forced to 1.0, symmetric/hermitian/skew files are expanded, and
+ * the result is row-major COO (rows ascending; inside a row, file order with
+ * mirrored entries placed when their source entry is read).  Arrays are
+ * malloc'ed; release them with cbspmv_free().  Negative returns -1..-7 are the
+ * reference reader's codes (open, banner, type, size, entry, expanded nnz
+ * overflow, allocation). */
+int cbspmv_read_mtx(const char* path, int* m, int* n, int* nnz, int** row, int** col, double** val);
+void cbspmv_free(void* p);
 
 /* ---- diagnostics ---------------------------------------------------------------- */
 const char* cbspmv_last_error(void);

@@ -15,7 +15,8 @@
 // are therefore ordered by calling std::sort with an equivalent comparator on
 // an identical input sequence.  Everything else is plain loops over arrays.
 //
-// How each part is pinned (tests/test_oracle_*.py):
+// How each part is pinned (tests/test_golden.py, test_builder_parity.py,
+// test_oracle.py):
 //   * builder arrays and payloads: equal to the reference builder compiled from
 //     its own sources (oracle/_ref), on the committed fixtures and on seeded
 //     random inputs; plus closed-form checks (floor-division block ids,

@@ -158,6 +158,10 @@ def ref_lib(bs=16, waves=4, row_owned=True, force_coo64=False):
         L.ref_array.argtypes = [vp, ctypes.c_char_p, vp]
         L.ref_check.restype = ctypes.c_int
         L.ref_check.argtypes = [ctypes.c_int, ip, ip, dp, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.ref_read_mtx.restype = ctypes.c_int
+        L.ref_read_mtx.argtypes = [ctypes.c_char_p, ip, ip, ip, ctypes.POINTER(ip), ctypes.POINTER(ip),
+                                   ctypes.POINTER(dp)]
+        L.ref_free_buf.argtypes = [vp]
         L.ref_time_check.restype = ctypes.c_double
         L.ref_time_check.argtypes = [ctypes.c_int, ip, ip, dp, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int]
         _ref_libs[path] = L
@@ -203,6 +207,24 @@ def ref_check(m, n, row, col, val, x, y):
     row, col, val, x, y = _i32(row), _i32(col), _f64(val), _f64(x), _f64(y)
     return bool(L.ref_check(int(row.size), _p(row, ctypes.c_int), _p(col, ctypes.c_int), _p(val, ctypes.c_double),
                             int(m), int(n), _p(x, ctypes.c_double), _p(y, ctypes.c_double)))
+
+
+def ref_read_mtx(path):
+    """The reference reader (read_matrix_coo_data, mmio_highlevel.h:313)."""
+    L = ref_lib()
+    ip = ctypes.POINTER(ctypes.c_int)
+    m, n, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    r, c, v = ip(), ip(), ctypes.POINTER(ctypes.c_double)()
+    rc = L.ref_read_mtx(str(path).encode(), ctypes.byref(m), ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(r),
+                        ctypes.byref(c), ctypes.byref(v))
+    if rc != 0:
+        raise RuntimeError(f"reference reader returned {rc}")
+    k = nnz.value
+    out = (m.value, n.value, np.ctypeslib.as_array(r, shape=(k,)).copy(),
+           np.ctypeslib.as_array(c, shape=(k,)).copy(), np.ctypeslib.as_array(v, shape=(k,)).copy())
+    for ptr in (r, c, v):
+        L.ref_free_buf(ctypes.cast(ptr, ctypes.c_void_p))
+    return out
 
 
 def rel_l2(y, y_ref):

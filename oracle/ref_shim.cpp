@@ -5,6 +5,7 @@
 // cpu_baseline leg may time check_results_consistency() from it.
 //
 // Wrapped reference entry points:
+//   read_matrix_coo_data         cb-spmv-dcu/src/io/mmio_highlevel.h:313
 //   GLOBAL_BLOCK ctor            cb-spmv-dcu/src/format.h:106
 //   coo2block_gather / _comp     cb-spmv-dcu/src/coo2block.h:628 / :605
 //   check_results_consistency    cb-spmv-dcu/src/utils.h:85
@@ -15,6 +16,7 @@
 
 #include "coo2block.h"
 #include "utils.h"
+#include "io/mmio_highlevel.h"
 
 namespace {
 template <typename T>
@@ -25,6 +27,28 @@ long copy_out(const std::vector<T>& v, void* dst) {
 }  // namespace
 
 extern "C" {
+
+// Reads a .mtx file with the reference reader; returns its status and the
+// row-major COO (caller frees the three arrays with ref_free_buf).
+int ref_read_mtx(const char* path, int* m, int* n, int* nnz, int** row, int** col, double** val) {
+    std::string type;
+    int M = 0, N = 0, L = 0;
+    int rc = read_matrix_info(path, type, M, N, L);
+    if (rc != 0) return rc;
+    std::vector<int> I, J;
+    std::vector<double> V;
+    rc = read_matrix_coo_data(path, I, J, V);
+    if (rc != 0) return rc;
+    *m = M; *n = N; *nnz = (int)I.size();
+    *row = (int*)malloc(sizeof(int) * (I.size() + 1));
+    *col = (int*)malloc(sizeof(int) * (I.size() + 1));
+    *val = (double*)malloc(sizeof(double) * (I.size() + 1));
+    std::copy(I.begin(), I.end(), *row);
+    std::copy(J.begin(), J.end(), *col);
+    std::copy(V.begin(), V.end(), *val);
+    return 0;
+}
+void ref_free_buf(void* p) { free(p); }
 
 int ref_block_size() { return BLOCK_SIZE; }
 int ref_waves() { return WARP_PER_BLOCK; }
