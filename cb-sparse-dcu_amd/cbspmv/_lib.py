@@ -28,6 +28,7 @@ BUILDER_GATHER, BUILDER_COMP = 0, 1
 F64, F32, BF16 = 0, 1, 2
 ACC_NATIVE, ACC_COMPENSATED = 0, 1
 SCHED_AUTO, SCHED_BLOCKED, SCHED_STREAM = 0, 1, 2
+BACKWARD_FUSED, BACKWARD_DIRECT = 0, 1
 
 
 class CbspmvError(RuntimeError):
@@ -53,7 +54,8 @@ class Opts(ctypes.Structure):
         ("stream_item_nnz", ctypes.c_int),
         ("stream_tile_waves", ctypes.c_int),
         ("stream_tile_kib", ctypes.c_int),
-        ("reserved", ctypes.c_int * 3),
+        ("backward_mode", ctypes.c_int),
+        ("reserved", ctypes.c_int * 2),
     ]
 
 
@@ -112,6 +114,12 @@ def lib():
         "cbspmv_last_error": (ctypes.c_char_p, []),
         "cbspmv_version": (ctypes.c_char_p, []),
         "cbspmv_last_kernel_ms": (ctypes.c_double, [vp]),
+        "cbspmv_build_transpose": (ctypes.c_int, [vp]),
+        "cbspmv_stored_entries": (ctypes.c_longlong, [vp]),
+        "cbspmv_spmv_transpose": (ctypes.c_int, [vp, vp, vp, vp]),
+        "cbspmv_backward": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+        "cbspmv_backward_stored": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+        "cbspmv_set_values": (ctypes.c_int, [vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -1,0 +1,70 @@
+"""torch.autograd wrapper over the C-ABI adjoint (extension: the reference
+computes y = A x only and has no gradients).
+
+    A = CBSparse(m, n, row, col, val, dtype="f32")      # builds A and A^T once
+    vals = torch.tensor(val, device="cuda", requires_grad=True)
+    y = cb_spmv(A, x, vals)          # y = A(vals) x
+    loss(y).backward()               # x.grad = A^T g, vals.grad[k] = g[row_k] x[col_k]
+
+The backward runs hand-written kernels only: dx over the transposed CB plan
+and dvals as the SDDMM over the stored pattern (cbspmv_backward).  Values
+are pushed to the device plans (cbspmv_set_values) when the tensor passed as
+`vals` changed since the last call (new storage or an in-place update).
+"""
+import torch
+
+from .plan import Plan
+
+
+class CBSparse:
+    """A sparse matrix held as a forward CB plan plus its transposed plan."""
+
+    def __init__(self, m, n, row, col, val, device=None, **opts):
+        self.plan = Plan(m, n, row, col, val, **opts)
+        self.plan.build_transpose()
+        self.m, self.n, self.nnz = self.plan.m, self.plan.n, self.plan.nnz
+        self.dtype = self.plan.vector_dtype
+        if device is not None:
+            with torch.cuda.device(device):
+                self.plan.upload()
+        else:
+            self.plan.upload()
+        self._synced = None
+
+    def sync_values(self, vals):
+        if vals.dtype != self.dtype or vals.numel() != self.nnz or vals.device.type != "cuda":
+            raise ValueError(f"vals must be a CUDA {self.dtype} tensor of {self.nnz} elements")
+        vals = vals.detach().contiguous()
+        key = (vals.data_ptr(), vals._version)
+        if key != self._synced:
+            self.plan.set_values(vals)
+            self._synced = key
+
+    def close(self):
+        self.plan.close()
+
+
+class _SpMV(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vals, x, mat):
+        if vals is not None:
+            mat.sync_values(vals)
+        x = x.contiguous()
+        y = mat.plan.spmv(x)
+        ctx.mat = mat
+        ctx.save_for_backward(x)
+        return y
+
+    @staticmethod
+    def backward(ctx, g):
+        (x,) = ctx.saved_tensors
+        want_vals, want_x = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        dx, dvals = ctx.mat.plan.backward(g.contiguous(), x, want_dx=want_x, want_dvals=want_vals)
+        return dvals, dx, None
+
+
+def cb_spmv(mat, x, vals=None):
+    """y = A x, differentiable in x and (when given) in A's values."""
+    if x.dim() != 1 or x.numel() != mat.n:
+        raise ValueError(f"x must be a vector of {mat.n} elements")
+    return _SpMV.apply(vals, x, mat)

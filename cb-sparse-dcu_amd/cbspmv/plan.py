@@ -29,6 +29,8 @@ _EXPORT_DTYPES = {
     "stream_tiles": np.uint32,
     "stream_tile_lines": np.uint32,
     "stream_empty_rows": np.uint32,
+    "grad_slot": np.uint32,
+    "slot_of": np.uint32,
 }
 ARRAY_NAMES = [
     "gb_row_idx", "gb_col_idx", "gb_row_ptr", "nnz_per_blk", "type_per_blk",
@@ -44,7 +46,7 @@ _SCHEDULES = {"auto": _lib.SCHED_AUTO, "blocked": _lib.SCHED_BLOCKED, "stream": 
 
 def make_opts(block_size=16, waves=4, row_owned=True, pattern_wave=True, force_coo64=False,
               dense_sixteenths=15, builder="gather", dtype="f64", compensated=False,
-              schedule="auto", item_nnz=0, tile_waves=0, tile_kib=0):
+              schedule="auto", item_nnz=0, tile_waves=0, tile_kib=0, backward="fused"):
     o = _lib.Opts()
     lib().cbspmv_default_opts(ctypes.byref(o))
     o.block_size = int(block_size)
@@ -60,6 +62,7 @@ def make_opts(block_size=16, waves=4, row_owned=True, pattern_wave=True, force_c
     o.stream_item_nnz = int(item_nnz)
     o.stream_tile_waves = int(tile_waves)
     o.stream_tile_kib = int(tile_kib)
+    o.backward_mode = {"fused": _lib.BACKWARD_FUSED, "direct": _lib.BACKWARD_DIRECT}[backward]
     return o
 
 
@@ -119,7 +122,7 @@ class Plan:
         """Returns the named plan array as a numpy array (see cbspmv.h)."""
         cnt = ctypes.c_size_t(0)
         check(lib().cbspmv_export(self._h, name.encode(), None, ctypes.byref(cnt)))
-        dt = _EXPORT_DTYPES.get(name, np.int32)
+        dt = _EXPORT_DTYPES.get(name.split(".", 1)[-1], np.int32)
         out = np.empty(cnt.value, dtype=dt)
         if cnt.value:
             check(lib().cbspmv_export(self._h, name.encode(), out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(cnt)))
@@ -161,6 +164,53 @@ class Plan:
         check(lib().cbspmv_spmv(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
                                 _stream_handle(stream)))
         return y
+
+    # ---- adjoint (extension: no reference counterpart)
+    def build_transpose(self):
+        """Builds A^T once (host); device copies follow on first use."""
+        check(lib().cbspmv_build_transpose(self._h))
+
+    def stored_entries(self):
+        return int(lib().cbspmv_stored_entries(self._h))
+
+    def spmv_transpose(self, g, dx=None, stream=None):
+        """dx = A^T g on the device."""
+        self._check_vec(g, self.m, "g")
+        if dx is None:
+            dx = torch.empty(self.n, dtype=self.vector_dtype, device=g.device)
+        self._check_vec(dx, self.n, "dx")
+        check(lib().cbspmv_spmv_transpose(self._h, ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(dx.data_ptr()),
+                                          _stream_handle(stream)))
+        return dx
+
+    def backward(self, g, x, want_dx=True, want_dvals=True, stream=None):
+        """(dx, dvals) of y = A x for upstream gradient g; dvals in input COO
+        order.  Either may be None when not wanted."""
+        self._check_vec(g, self.m, "g")
+        self._check_vec(x, self.n, "x")
+        dx = torch.empty(self.n, dtype=self.vector_dtype, device=g.device) if want_dx else None
+        dv = torch.empty(self.nnz, dtype=self.vector_dtype, device=g.device) if want_dvals else None
+        check(lib().cbspmv_backward(self._h, ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                    ctypes.c_void_p(dx.data_ptr() if dx is not None else 0),
+                                    ctypes.c_void_p(dv.data_ptr() if dv is not None else 0), _stream_handle(stream)))
+        return dx, dv
+
+    def backward_stored(self, g, x, dvals, dx=None, stream=None):
+        """Fused pass: dx (optional) and dvals in stored order."""
+        self._check_vec(g, self.m, "g")
+        self._check_vec(x, self.n, "x")
+        self._check_vec(dvals, self.stored_entries(), "dvals")
+        check(lib().cbspmv_backward_stored(self._h, ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                           ctypes.c_void_p(dx.data_ptr() if dx is not None else 0),
+                                           ctypes.c_void_p(dvals.data_ptr()), _stream_handle(stream)))
+        return dx, dvals
+
+    def set_values(self, vals, stream=None):
+        """Replaces A's values (device tensor, input COO order)."""
+        if not self._uploaded:
+            self.upload(stream)
+        self._check_vec(vals, self.nnz, "vals")
+        check(lib().cbspmv_set_values(self._h, ctypes.c_void_p(vals.data_ptr()), _stream_handle(stream)))
 
     def bench(self, x):
         """cb_spmv() protocol on host FP64 x: returns (y, runtime_ms, gflops)."""

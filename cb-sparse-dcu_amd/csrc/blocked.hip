@@ -247,6 +247,13 @@ __global__ void zero_rows(double* y, int m, const uint32_t* strips, uint32_t n_s
     }
 }
 
+// Zero `words` 32-bit words (a kernel rather than hipMemsetAsync so that the
+// call can sit inside a captured graph).
+__global__ void zero_words(uint32_t* p, size_t words) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < words; t += (size_t)gridDim.x * blockDim.x)
+        p[t] = 0u;
+}
+
 // One wave per distinct row pattern of a unit-valued matrix.
 template <bool UNROLLED>
 __global__ void pattern_spmv(PatternArgs a) {
@@ -289,6 +296,13 @@ hipError_t launch_zero_rows(double* y, int m, const uint32_t* strips, uint32_t n
     if (total == 0) return hipSuccess;
     const int grid = (int)std::min<size_t>((total + 255) / 256, 4096);
     hipLaunchKernelGGL(dev::zero_rows, dim3(grid), dim3(256), 0, s, y, m, strips, n, bs);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_words(void* p, size_t words, hipStream_t s) {
+    if (words == 0) return hipSuccess;
+    const int grid = (int)std::min<size_t>((words + 255) / 256, 4096);
+    hipLaunchKernelGGL(dev::zero_words, dim3(grid), dim3(256), 0, s, static_cast<uint32_t*>(p), words);
     return hipGetLastError();
 }
 

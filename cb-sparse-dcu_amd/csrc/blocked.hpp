@@ -40,6 +40,7 @@ struct PatternArgs {
 
 hipError_t launch_blocked(int bs, const BlockedArgs& a, int grid, hipStream_t s);
 hipError_t launch_zero_rows(double* y, int m, const uint32_t* strips, uint32_t n, int bs, hipStream_t s);
+hipError_t launch_zero_words(void* p, size_t words, hipStream_t s);
 hipError_t launch_pattern(const PatternArgs& a, bool unrolled, int waves, hipStream_t s);
 
 }  // namespace cbs

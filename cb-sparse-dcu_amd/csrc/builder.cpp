@@ -237,6 +237,7 @@ int build_plan(Plan& p, int m, int n, int nnz, const int* row, const int* col,
     }
     p.in_row.assign(row, row + nnz);
     p.in_col.assign(col, col + nnz);
+    p.in_val.assign(val, val + nnz);
 
     // entry range of each block row
     std::vector<int64_t> br_ptr(p.gb_m + 1, 0);

@@ -1,5 +1,6 @@
-// C ABI, host half: options, plan build/destroy, statistics and parity export.
-// The device half (upload, SpMV, bench, backward, batching) is device.hip.
+// C ABI, host half: options, plan build, statistics and parity export.  The
+// device half is runtime.hip (upload, SpMV, bench) and adjoint.hip
+// (transposed plan, backward, value replacement).
 #include <cstring>
 #include <new>
 #include <string>
@@ -16,6 +17,70 @@ int fail(int code, const std::string& msg) {
 }  // namespace cbs
 
 using cbs::fail;
+
+namespace cbs {
+int build_plan_object(const cbspmv_opts& o, int m, int n, int nnz, const int* row, const int* col,
+                      const double* val, bool canonical, cbspmv_plan** out) {
+    *out = nullptr;
+    const int bs = o.block_size;
+    if (bs != 2 && bs != 4 && bs != 8 && bs != 16 && bs != 32 && bs != 64)
+        return fail(CBSPMV_ERR_ARG, "block_size must be 2, 4, 8, 16, 32 or 64");
+    if (o.waves_per_block < 1 || o.waves_per_block > 16)
+        return fail(CBSPMV_ERR_ARG, "waves_per_block must be in 1..16");
+    if (o.dense_sixteenths < 1 || o.dense_sixteenths > 16)
+        return fail(CBSPMV_ERR_ARG, "dense_sixteenths must be in 1..16");
+    if (o.builder != CBSPMV_BUILDER_GATHER && o.builder != CBSPMV_BUILDER_COMP)
+        return fail(CBSPMV_ERR_ARG, "unknown builder");
+    if (o.value_dtype < CBSPMV_F64 || o.value_dtype > CBSPMV_BF16)
+        return fail(CBSPMV_ERR_ARG, "unknown value dtype");
+    if (o.schedule < CBSPMV_SCHED_AUTO || o.schedule > CBSPMV_SCHED_STREAM)
+        return fail(CBSPMV_ERR_ARG, "unknown schedule");
+    // stream keys hold column + row-in-strip + 2 flag bits in 32 bits
+    if ((long long)n > (1ll << (30 - coord_bits(bs))))
+        return fail(CBSPMV_ERR_UNSUPPORTED, "n above 2^(30 - log2 block_size) is not supported");
+
+    cbspmv_plan* p = new (std::nothrow) cbspmv_plan();
+    if (!p) return fail(CBSPMV_ERR_NOMEM, "out of host memory");
+    p->opts = o;
+    Plan& h = p->host;
+    h.o.bs = bs;
+    h.o.waves = o.waves_per_block;
+    h.o.row_owned = o.row_owned ? 1 : 0;
+    h.o.pattern_wave = o.pattern_wave ? 1 : 0;
+    h.o.force_coo64 = o.force_coo64 ? 1 : 0;
+    h.o.dense_sixteenths = o.dense_sixteenths;
+    h.o.builder = o.builder;
+    h.o.dtype = o.value_dtype;
+    h.o.accum = o.accumulate;
+    h.o.schedule = canonical ? o.schedule : CBSPMV_SCHED_STREAM;
+    h.o.item_nnz = o.stream_item_nnz;
+    if (o.stream_tile_waves) h.o.tile_items = o.stream_tile_waves;
+    if (o.stream_tile_kib) h.o.tile_bytes = o.stream_tile_kib * 1024;
+    if (h.o.tile_items < 1 || h.o.tile_items > kMaxTileItems || h.o.tile_bytes < 4096 || h.o.tile_bytes > 160 * 1024) {
+        delete p;
+        return fail(CBSPMV_ERR_ARG, "stream tile shape out of range (1..16 waves, 4..160 KiB)");
+    }
+    std::string err;
+    int rc;
+    try {
+        rc = build_plan(h, m, n, nnz, row, col, val, err);
+        if (rc == CBSPMV_OK) {
+            if (canonical) pack_canonical(h);
+            build_stream(h);
+        }
+    } catch (const std::bad_alloc&) {
+        delete p;
+        return fail(CBSPMV_ERR_NOMEM, "out of host memory while building the plan");
+    }
+    if (rc != CBSPMV_OK) {
+        delete p;
+        return fail(rc, err);
+    }
+    p->schedule = canonical ? choose_schedule(*p) : CBSPMV_SCHED_STREAM;
+    *out = p;
+    return CBSPMV_OK;
+}
+}  // namespace cbs
 
 extern "C" {
 
@@ -43,63 +108,7 @@ int cbspmv_build(int m, int n, int nnz, const int* row, const int* col, const do
     if (nnz > 0 && (!row || !col || !val)) return fail(CBSPMV_ERR_ARG, "null COO array");
     cbspmv_opts d;
     cbspmv_default_opts(&d);
-    const cbspmv_opts& o = opts ? *opts : d;
-    const int bs = o.block_size;
-    if (bs != 2 && bs != 4 && bs != 8 && bs != 16 && bs != 32 && bs != 64)
-        return fail(CBSPMV_ERR_ARG, "block_size must be 2, 4, 8, 16, 32 or 64");
-    if (o.waves_per_block < 1 || o.waves_per_block > 16)
-        return fail(CBSPMV_ERR_ARG, "waves_per_block must be in 1..16");
-    if (o.dense_sixteenths < 1 || o.dense_sixteenths > 16)
-        return fail(CBSPMV_ERR_ARG, "dense_sixteenths must be in 1..16");
-    if (o.builder != CBSPMV_BUILDER_GATHER && o.builder != CBSPMV_BUILDER_COMP)
-        return fail(CBSPMV_ERR_ARG, "unknown builder");
-    if (o.value_dtype < CBSPMV_F64 || o.value_dtype > CBSPMV_BF16)
-        return fail(CBSPMV_ERR_ARG, "unknown value dtype");
-    if (o.schedule < CBSPMV_SCHED_AUTO || o.schedule > CBSPMV_SCHED_STREAM)
-        return fail(CBSPMV_ERR_ARG, "unknown schedule");
-    // stream keys hold column + row-in-strip + 2 flag bits in 32 bits
-    if ((long long)n > (1ll << (30 - cbs::coord_bits(bs))))
-        return fail(CBSPMV_ERR_UNSUPPORTED, "n above 2^(30 - log2 block_size) is not supported");
-
-    cbspmv_plan* p = new (std::nothrow) cbspmv_plan();
-    if (!p) return fail(CBSPMV_ERR_NOMEM, "out of host memory");
-    cbs::Plan& h = p->host;
-    h.o.bs = bs;
-    h.o.waves = o.waves_per_block;
-    h.o.row_owned = o.row_owned ? 1 : 0;
-    h.o.pattern_wave = o.pattern_wave ? 1 : 0;
-    h.o.force_coo64 = o.force_coo64 ? 1 : 0;
-    h.o.dense_sixteenths = o.dense_sixteenths;
-    h.o.builder = o.builder;
-    h.o.dtype = o.value_dtype;
-    h.o.accum = o.accumulate;
-    h.o.schedule = o.schedule;
-    h.o.item_nnz = o.stream_item_nnz;
-    if (o.stream_tile_waves) h.o.tile_items = o.stream_tile_waves;
-    if (o.stream_tile_kib) h.o.tile_bytes = o.stream_tile_kib * 1024;
-    if (h.o.tile_items < 1 || h.o.tile_items > cbs::kMaxTileItems || h.o.tile_bytes < 4096 || h.o.tile_bytes > 160 * 1024) {
-        delete p;
-        return fail(CBSPMV_ERR_ARG, "stream tile shape out of range (1..16 waves, 4..160 KiB)");
-    }
-    std::string err;
-    int rc;
-    try {
-        rc = cbs::build_plan(h, m, n, nnz, row, col, val, err);
-        if (rc == CBSPMV_OK) {
-            cbs::pack_canonical(h);
-            cbs::build_stream(h);
-        }
-    } catch (const std::bad_alloc&) {
-        delete p;
-        return fail(CBSPMV_ERR_NOMEM, "out of host memory while building the plan");
-    }
-    if (rc != CBSPMV_OK) {
-        delete p;
-        return fail(rc, err);
-    }
-    p->schedule = cbs::choose_schedule(*p);
-    *plan = p;
-    return CBSPMV_OK;
+    return cbs::build_plan_object(opts ? *opts : d, m, n, nnz, row, col, val, true, plan);
 }
 
 int cbspmv_plan_stats(const cbspmv_plan* plan, cbspmv_stats* s) {
@@ -133,6 +142,16 @@ int cbspmv_plan_stats(const cbspmv_plan* plan, cbspmv_stats* s) {
 
 int cbspmv_export(const cbspmv_plan* plan, const char* name, void* dst, size_t* count) {
     if (!plan || !name || !count) return fail(CBSPMV_ERR_ARG, "null argument");
+    if (std::strncmp(name, "transpose.", 10) == 0) {
+        if (!plan->transpose) return fail(CBSPMV_ERR_STATE, "no transposed plan");
+        return cbspmv_export(plan->transpose, name + 10, dst, count);
+    }
+    if (std::strcmp(name, "grad_slot") == 0 || std::strcmp(name, "slot_of") == 0) {
+        const std::vector<uint32_t>& v = name[0] == 'g' ? plan->grad_slot : plan->slot_of;
+        if (dst && *count >= v.size() && !v.empty()) std::memcpy(dst, v.data(), v.size() * 4);
+        *count = v.size();
+        return CBSPMV_OK;
+    }
     int status = CBSPMV_OK;
     const long c = cbs::export_array(plan->host, name, dst, *count, status);
     if (status != CBSPMV_OK) return fail(status, std::string("unknown export name: ") + name);

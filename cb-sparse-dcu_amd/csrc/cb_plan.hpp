@@ -121,8 +121,10 @@ struct Plan {
     std::vector<int64_t> csr_rp_start;
     std::vector<double> dense_val;   // bs*bs per DENSE block
 
-    // Input COO kept for the SDDMM / value updates (row-major, input order).
+    // Input COO kept for the transposed build, the SDDMM and value updates
+    // (row-major, input order).
     std::vector<int> in_row, in_col;
+    std::vector<double> in_val;
 
     // Canonical packed slab (cb-spmv-dcu.hpp:1130-1348), block addresses as
     // byte offsets.  Built on demand.
@@ -154,5 +156,11 @@ int build_plan(Plan& p, int m, int n, int nnz, const int* row, const int* col,
 void pack_canonical(Plan& p);
 void build_stream(Plan& p);
 long export_array(const Plan& p, const std::string& name, void* dst, size_t cap, int& status);
+// For every input entry k (in_row/in_col order), its index in the padded
+// entry stream (duplicates merged into one stream entry share it).  Returns
+// false if the stream does not hold the input's entries.
+bool stream_entry_slots(const Plan& p, std::vector<uint32_t>& slot);
+// Byte offset in s_slab of the value of padded stream entry e.
+size_t stream_value_offset(const Plan& p, uint32_t item, uint32_t e);
 
 }  // namespace cbs

@@ -452,6 +452,97 @@ void build_stream(Plan& p) {
     p.stream_ready = true;
 }
 
+// ---- stream entry <-> input entry ---------------------------------------------
+size_t stream_value_offset(const Plan& p, uint32_t item, uint32_t e) {
+    const int epl = stream_lane_entries(p.o.dtype);
+    const size_t vb = (size_t)stream_val_bytes(p.o.dtype);
+    const size_t eb = 4 + vb;
+    const int nk = epl * 4 / 16;
+    const StreamItem& it = p.items[item];
+    const uint32_t chunk = (uint32_t)(64 * epl);
+    const uint32_t e0 = it.e_begin + (e - it.e_begin) / chunk * chunk;
+    const uint32_t G = std::min<uint32_t>(64u, (it.e_end - e0) / (uint32_t)epl);
+    const uint32_t l = (e - e0) / (uint32_t)epl, j = (e - e0) % (uint32_t)epl;
+    return (size_t)e0 * eb + (size_t)(nk + j * vb / 16) * G * 16 + (size_t)l * 16 + (j * vb) % 16;
+}
+
+// Decodes (row, column) of every real stream entry from its key, then pairs
+// input entries with stream entries row by row on the column: a column that
+// occurs once in the stream row takes every input entry of that (row, col)
+// (COO and DENSE blocks merged them); a column that occurs as often in the
+// stream row as in the input row (CSR blocks keep duplicates, in input
+// order) is paired in order.
+bool stream_entry_slots(const Plan& p, std::vector<uint32_t>& slot) {
+    const int64_t nnz = p.nnz;
+    slot.assign((size_t)nnz, 0u);
+    if (nnz == 0) return true;
+    const int cb = p.col_bits;
+    const uint32_t cmask = (1u << cb) - 1u, omask = (1u << (31 - cb)) - 1u;
+    const uint32_t zero_slot = (uint32_t)tile_slots(p.o.tile_bytes, p.o.dtype) - 1u;
+    const size_t NI = p.items.size();
+    // real entries in stream order; rows are non-decreasing along the stream
+    std::vector<uint32_t> srow, scol, spos;
+    srow.reserve((size_t)nnz);
+    scol.reserve((size_t)nnz);
+    spos.reserve((size_t)nnz);
+    for (size_t i = 0; i < NI; ++i) {
+        const StreamItem& it = p.items[i];
+        const size_t t = i / (size_t)p.o.tile_items;
+        for (uint32_t e = it.e_begin; e < it.e_end; ++e) {
+            const uint32_t k = p.s_key[e];
+            const uint32_t low = k & cmask;
+            const bool cached = (k & kKeyCached) != 0;
+            if (cached && low == zero_slot) continue;   // padding
+            const uint32_t col = cached ? p.tile_lines[p.tiles[2 * t] + low / kLineElems] * kLineElems + low % kLineElems : low;
+            srow.push_back(it.row_base + ((k >> cb) & omask));
+            scol.push_back(col);
+            spos.push_back(e);
+        }
+    }
+    const int m = p.m;
+    std::vector<int64_t> sptr((size_t)m + 1, 0), iptr((size_t)m + 1, 0);
+    for (uint32_t r : srow) {
+        if (r >= (uint32_t)m) return false;
+        sptr[r + 1]++;
+    }
+    for (int r : p.in_row) iptr[r + 1]++;
+    for (int r = 0; r < m; ++r) { sptr[r + 1] += sptr[r]; iptr[r + 1] += iptr[r]; }
+    for (size_t q = 1; q < srow.size(); ++q) if (srow[q] < srow[q - 1]) return false;
+    bool ok = true;
+#pragma omp parallel
+    {
+        std::vector<std::pair<uint32_t, uint32_t>> sc, ic;   // (col, position) / (col, input index)
+#pragma omp for schedule(dynamic, 256) reduction(&& : ok)
+        for (int r = 0; r < m; ++r) {
+            sc.clear(); ic.clear();
+            for (int64_t q = sptr[r]; q < sptr[r + 1]; ++q) sc.push_back({scol[q], spos[q]});
+            for (int64_t q = iptr[r]; q < iptr[r + 1]; ++q) ic.push_back({(uint32_t)p.in_col[q], (uint32_t)q});
+            std::sort(sc.begin(), sc.end());
+            std::sort(ic.begin(), ic.end());
+            size_t a = 0, b = 0;
+            while (b < ic.size()) {
+                const uint32_t c = ic[b].first;
+                while (a < sc.size() && sc[a].first < c) ++a;   // stream-only (zero) entries
+                size_t a2 = a, b2 = b;
+                while (a2 < sc.size() && sc[a2].first == c) ++a2;
+                while (b2 < ic.size() && ic[b2].first == c) ++b2;
+                const size_t ns = a2 - a, ni = b2 - b;
+                if (ns == 1) {
+                    for (size_t q = b; q < b2; ++q) slot[ic[q].second] = sc[a].second;
+                } else if (ns == ni) {
+                    for (size_t q = 0; q < ni; ++q) slot[ic[b + q].second] = sc[a + q].second;
+                } else {
+                    ok = false;
+                    break;
+                }
+                a = a2;
+                b = b2;
+            }
+        }
+    }
+    return ok;
+}
+
 // ---- parity export -----------------------------------------------------------
 namespace {
 template <typename T>

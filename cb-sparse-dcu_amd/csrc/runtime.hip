@@ -13,17 +13,9 @@
 
 namespace cbs {
 
-namespace {
-
 int hip_fail(hipError_t e, const char* what) {
     return fail(CBSPMV_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
-
-#define CBS_HIP(call)                                   \
-    do {                                                \
-        const hipError_t e_ = (call);                   \
-        if (e_ != hipSuccess) return hip_fail(e_, #call); \
-    } while (0)
 
 void release(DeviceBuf& b) {
     if (b.ptr) (void)hipFree(b.ptr);
@@ -42,20 +34,12 @@ int upload_bytes(DeviceBuf& b, const void* src, size_t bytes, hipStream_t s) {
     return CBSPMV_OK;
 }
 
-template <typename T>
-int upload_vec(DeviceBuf& b, const std::vector<T>& v, hipStream_t s) {
-    return upload_bytes(b, v.empty() ? nullptr : v.data(), v.size() * sizeof(T), s);
-}
-
 size_t vec_elem_bytes(const Plan& h) { return h.o.dtype == CBSPMV_F64 ? 8 : 4; }
-size_t val_elem_bytes(const Plan& h) { return h.o.dtype == CBSPMV_F64 ? 8 : h.o.dtype == CBSPMV_F32 ? 4 : 2; }
-
-}  // namespace
 
 void release_device(DevicePlan& d) {
     for (DeviceBuf* b : {&d.slab, &d.zero_strips, &d.key, &d.val, &d.items, &d.item_split,
                          &d.item_split_first, &d.splits, &d.partial, &d.counters, &d.empty, &d.tiles, &d.tile_lines,
-                         &d.in_row, &d.in_col})
+                         &d.sv_pos, &d.sv_ptr, &d.sv_src, &d.grad_slot, &d.dvals_stored, &d.in_row, &d.in_col})
         release(*b);
     d.uploaded = false;
 }
@@ -124,7 +108,7 @@ int stream_grid(const Plan& h) {
     return (int)std::max<long long>(8, (tiles + 7) / 8 * 8);
 }
 
-int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s) {
+int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s, const void* xrow, void* dvals) {
     const Plan& h = p.host;
     const DevicePlan& d = p.dev;
     const int grid = stream_grid(h);
@@ -154,21 +138,27 @@ int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s) {
         a.partial = static_cast<double*>(d.partial.ptr);
         a.x = static_cast<const double*>(x);
         a.y = static_cast<double*>(y);
-        e = launch_stream(a, grid, s);
+        a.xrow = static_cast<const double*>(xrow);
+        a.dvals = static_cast<double*>(dvals);
+        e = launch_stream(a, grid, dvals != nullptr, s);
     } else if (h.o.dtype == CBSPMV_F32) {
         StreamArgs<float, float> a{};
         fill(a);
         a.partial = static_cast<float*>(d.partial.ptr);
         a.x = static_cast<const float*>(x);
         a.y = static_cast<float*>(y);
-        e = launch_stream(a, grid, h.o.accum == CBSPMV_ACC_COMPENSATED, s);
+        a.xrow = static_cast<const float*>(xrow);
+        a.dvals = static_cast<float*>(dvals);
+        e = launch_stream(a, grid, h.o.accum == CBSPMV_ACC_COMPENSATED, dvals != nullptr, s);
     } else {
         StreamArgs<dev::bf16_t, float> a{};
         fill(a);
         a.partial = static_cast<float*>(d.partial.ptr);
         a.x = static_cast<const float*>(x);
         a.y = static_cast<float*>(y);
-        e = launch_stream(a, grid, h.o.accum == CBSPMV_ACC_COMPENSATED, s);
+        a.xrow = static_cast<const float*>(xrow);
+        a.dvals = static_cast<float*>(dvals);
+        e = launch_stream(a, grid, h.o.accum == CBSPMV_ACC_COMPENSATED, dvals != nullptr, s);
     }
     if (e != hipSuccess) return hip_fail(e, "stream schedule launch");
     return CBSPMV_OK;
@@ -237,10 +227,11 @@ int run_spmv(cbspmv_plan& p, const void* x, void* y, hipStream_t s) {
     const Plan& h = p.host;
     if (h.m == 0) return CBSPMV_OK;
     if (h.nnz == 0) {
-        CBS_HIP(hipMemsetAsync(y, 0, (size_t)h.m * vec_elem_bytes(h), s));
-        return CBSPMV_OK;
+        const hipError_t e = launch_zero_words(y, (size_t)h.m * vec_elem_bytes(h) / 4, s);
+        return e == hipSuccess ? CBSPMV_OK : hip_fail(e, "zero launch");
     }
-    const int sched = p.forced != CBSPMV_SCHED_AUTO ? p.forced : p.schedule;
+    int sched = p.forced != CBSPMV_SCHED_AUTO ? p.forced : p.schedule;
+    if (p.values_replaced) sched = CBSPMV_SCHED_STREAM;   // only the stream slab takes new values
     return sched == CBSPMV_SCHED_BLOCKED ? run_blocked(p, x, y, s) : run_stream(p, x, y, s);
 }
 
@@ -274,12 +265,15 @@ int cbspmv_set_schedule(cbspmv_plan* plan, int schedule) {
     if (schedule < CBSPMV_SCHED_AUTO || schedule > CBSPMV_SCHED_STREAM) return fail(CBSPMV_ERR_ARG, "unknown schedule");
     if (schedule == CBSPMV_SCHED_BLOCKED && plan->host.o.dtype != CBSPMV_F64)
         return fail(CBSPMV_ERR_UNSUPPORTED, "the blocked schedule is FP64 only");
+    if (schedule == CBSPMV_SCHED_BLOCKED && plan->values_replaced)
+        return fail(CBSPMV_ERR_UNSUPPORTED, "values were replaced: only the stream schedule holds them");
     plan->forced = schedule;
     return CBSPMV_OK;
 }
 
 int cbspmv_get_schedule(const cbspmv_plan* plan) {
     if (!plan) return fail(CBSPMV_ERR_ARG, "null plan");
+    if (plan->values_replaced) return CBSPMV_SCHED_STREAM;
     return plan->forced != CBSPMV_SCHED_AUTO ? plan->forced : plan->schedule;
 }
 

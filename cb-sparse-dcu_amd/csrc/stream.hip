@@ -13,6 +13,11 @@
 // need no atomics; only a row longer than four items is cut, and its pieces
 // meet in a partial buffer where the last piece to arrive (atomic ticket) sums
 // them in piece order.  Results are the same bits on every launch.
+//
+// The SDDMM variant runs on the transposed plan of a backward pass: besides
+// dx = A^T g it writes, for every stream entry (row j of A^T, column i),
+// dvals = g[i] * x[j] in stream order, reusing the g[i] the product gathers
+// anyway; lanes own consecutive entries, so those stores are coalesced.
 #include <type_traits>
 
 #include "kernels.hpp"
@@ -105,7 +110,7 @@ __device__ __forceinline__ double wave_shr1(double v) { return dpp<kWaveShr1, 0x
 __device__ __forceinline__ float wave_shr1(float v) { return dpp<kWaveShr1, 0xf>(v); }
 __device__ __forceinline__ f2 wave_shr1(f2 v) { return f2{dpp<kWaveShr1, 0xf>(v.hi), dpp<kWaveShr1, 0xf>(v.lo)}; }
 
-template <typename V, typename A, bool COMP, int EPL>
+template <typename V, typename A, bool COMP, int EPL, bool SDDMM>
 __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A> a) {
     using Acc = typename std::conditional<COMP, f2, A>::type;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -214,6 +219,23 @@ __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A
             if constexpr (COMP) s[j] = two_prod(widen(cur.v[j]), xc[j]);
             else s[j] = A(widen(cur.v[j])) * xc[j];
         }
+        if constexpr (SDDMM) {
+            if (valid) {
+                A dv[EPL];
+#pragma unroll
+                for (int j = 0; j < EPL; ++j) {
+                    const uint32_t r = min(it.row_base + (uint32_t)off[j], (uint32_t)a.m - 1u);
+                    dv[j] = xc[j] * a.xrow[r];
+                }
+                u32x4* dst = reinterpret_cast<u32x4*>(a.dvals + e0 + (uint32_t)lane * EPL);
+#pragma unroll
+                for (int q = 0; q < EPL * (int)sizeof(A) / 16; ++q) {
+                    u32x4 w;
+                    __builtin_memcpy(&w, reinterpret_cast<const unsigned char*>(dv) + 16 * q, 16);
+                    __builtin_nontemporal_store(w, dst + q);
+                }
+            }
+        }
         // the run left open by the previous chunk
         if (carry_off >= 0) {
             const int o0 = __builtin_amdgcn_readfirstlane(off[0]);
@@ -278,10 +300,10 @@ __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A
 
 }  // namespace dev
 
-template <typename V, typename A, bool COMP>
+template <typename V, typename A, bool COMP, bool SDDMM>
 hipError_t launch_stream_t(const StreamArgs<V, A>& a, int grid, hipStream_t s) {
     constexpr int EPL = sizeof(V) >= 8 ? 4 : 8;
-    auto kern = dev::stream_spmv<V, A, COMP, EPL>;
+    auto kern = dev::stream_spmv<V, A, COMP, EPL, SDDMM>;
     const size_t lds = (size_t)a.tile_slots * sizeof(A);
     static bool attr_set = false;   // allow more than 64 KB of dynamic LDS
     if (!attr_set) {
@@ -293,15 +315,21 @@ hipError_t launch_stream_t(const StreamArgs<V, A>& a, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_stream(const StreamArgs<double, double>& a, int grid, hipStream_t s) {
-    return launch_stream_t<double, double, false>(a, grid, s);
+template <typename V, typename A>
+hipError_t launch_stream_any(const StreamArgs<V, A>& a, int grid, bool comp, bool sddmm, hipStream_t s) {
+    if (comp) return sddmm ? launch_stream_t<V, A, true, true>(a, grid, s) : launch_stream_t<V, A, true, false>(a, grid, s);
+    return sddmm ? launch_stream_t<V, A, false, true>(a, grid, s) : launch_stream_t<V, A, false, false>(a, grid, s);
 }
-hipError_t launch_stream(const StreamArgs<float, float>& a, int grid, bool comp, hipStream_t s) {
-    return comp ? launch_stream_t<float, float, true>(a, grid, s) : launch_stream_t<float, float, false>(a, grid, s);
+
+hipError_t launch_stream(const StreamArgs<double, double>& a, int grid, bool sddmm, hipStream_t s) {
+    return sddmm ? launch_stream_t<double, double, false, true>(a, grid, s)
+                 : launch_stream_t<double, double, false, false>(a, grid, s);
 }
-hipError_t launch_stream(const StreamArgs<dev::bf16_t, float>& a, int grid, bool comp, hipStream_t s) {
-    return comp ? launch_stream_t<dev::bf16_t, float, true>(a, grid, s)
-                : launch_stream_t<dev::bf16_t, float, false>(a, grid, s);
+hipError_t launch_stream(const StreamArgs<float, float>& a, int grid, bool comp, bool sddmm, hipStream_t s) {
+    return launch_stream_any(a, grid, comp, sddmm, s);
+}
+hipError_t launch_stream(const StreamArgs<dev::bf16_t, float>& a, int grid, bool comp, bool sddmm, hipStream_t s) {
+    return launch_stream_any(a, grid, comp, sddmm, s);
 }
 
 }  // namespace cbs

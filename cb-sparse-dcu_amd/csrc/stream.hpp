@@ -33,10 +33,15 @@ struct StreamArgs {
     int m, n;
     const A* x;
     A* y;
+    // backward over the transposed plan: dvals[e] = x[col(e)] * xrow[row(e)]
+    // for every padded stream entry e (SDDMM launches only)
+    const A* xrow;
+    A* dvals;
 };
 
-hipError_t launch_stream(const StreamArgs<double, double>& a, int grid, hipStream_t s);
-hipError_t launch_stream(const StreamArgs<float, float>& a, int grid, bool comp, hipStream_t s);
-hipError_t launch_stream(const StreamArgs<dev::bf16_t, float>& a, int grid, bool comp, hipStream_t s);
+// sddmm: also write dvals (a.xrow, a.dvals set)
+hipError_t launch_stream(const StreamArgs<double, double>& a, int grid, bool sddmm, hipStream_t s);
+hipError_t launch_stream(const StreamArgs<float, float>& a, int grid, bool comp, bool sddmm, hipStream_t s);
+hipError_t launch_stream(const StreamArgs<dev::bf16_t, float>& a, int grid, bool comp, bool sddmm, hipStream_t s);
 
 }  // namespace cbs

@@ -51,6 +51,11 @@ extern "C" {
 enum cbspmv_builder { CBSPMV_BUILDER_GATHER = 0, CBSPMV_BUILDER_COMP = 1 };
 enum cbspmv_dtype { CBSPMV_F64 = 0, CBSPMV_F32 = 1, CBSPMV_BF16 = 2 };
 enum cbspmv_accum { CBSPMV_ACC_NATIVE = 0, CBSPMV_ACC_COMPENSATED = 1 };
+enum cbspmv_backward_mode {
+    CBSPMV_BACKWARD_FUSED = 0, /* one pass over A^T writes dx and stored-order dvals,
+                                  then a permutation pass to input order          */
+    CBSPMV_BACKWARD_DIRECT = 1 /* dx over A^T; dvals by a pass over the input COO  */
+};
 enum cbspmv_schedule {
     CBSPMV_SCHED_AUTO = 0,    /* pick by plan size / autotuner                       */
     CBSPMV_SCHED_BLOCKED = 1, /* one wave per packed block over the canonical slab   */
@@ -71,7 +76,8 @@ typedef struct cbspmv_opts {
     int stream_item_nnz;  /* stream schedule: target entries per wave item (0=auto)  */
     int stream_tile_waves;/* stream schedule: waves per workgroup, 1..16 (0=auto: 8)   */
     int stream_tile_kib;  /* stream schedule: LDS x cache per workgroup, KiB (0=96)  */
-    int reserved[3];
+    int backward_mode;    /* enum cbspmv_backward_mode (cbspmv_backward)              */
+    int reserved[2];
 } cbspmv_opts;
 
 /* Builder statistics: the GLOBAL_BLOCK scalars (format.h:63-77) plus counts. */
@@ -144,6 +150,41 @@ int cbspmv_get_schedule(const cbspmv_plan* plan);
  * timed launches between hipEvents; runtime = ms / 1000, GFLOPS = 2 nnz / t.
  * x, y are host arrays of n and m doubles.  Timings go to *metrics. */
 int cbspmv_bench(cbspmv_plan* plan, const double* x, double* y, cbspmv_metrics* metrics);
+
+/* ---- differentiation (no reference counterpart; extension) ------------------ */
+/* Builds the transposed CB structure once (A^T from the input COO reordered
+ * column-major, stable within a column; same options) and the maps between
+ * input entries and stream entries.  Host only; the device copy is made on
+ * the first transposed call. */
+int cbspmv_build_transpose(cbspmv_plan* plan);
+/* Number of entries of the "stored order" (the transposed plan's padded
+ * stream), the length of cbspmv_backward_stored's dvals; -1 before
+ * cbspmv_build_transpose.  cbspmv_export(plan, "grad_slot") gives, for every
+ * input entry k, its stored-order index (uint32), "slot_of" its index in the
+ * plan's own stream, and "transpose.<name>" exports array <name> of A^T. */
+long long cbspmv_stored_entries(const cbspmv_plan* plan);
+/* d_dx = A^T d_g  (d_g: m elements, d_dx: n, plan's vector type). */
+int cbspmv_spmv_transpose(cbspmv_plan* plan, const void* d_g, void* d_dx, void* stream);
+/* Gradients of y = A x for an upstream gradient g = dL/dy:
+ *   d_dx    = A^T g                       (n elements; skipped when NULL)
+ *   d_dvals = g[row[k]] * x[col[k]]       (nnz elements, input COO order;
+ *                                          skipped when NULL)
+ * duplicates of one (row, col) each get the full product.  Mode from
+ * opts.backward_mode. */
+int cbspmv_backward(cbspmv_plan* plan, const void* d_g, const void* d_x,
+                    void* d_dx, void* d_dvals, void* stream);
+/* The fused pass alone: d_dx (may be NULL) and dvals in stored order
+ * (cbspmv_stored_entries() elements, 16-byte aligned); stored entries that
+ * hold no input entry (zero fill of dense blocks, padding) get g_i x_j or 0. */
+int cbspmv_backward_stored(cbspmv_plan* plan, const void* d_g, const void* d_x,
+                           void* d_dx, void* d_dvals_stored, void* stream);
+/* Replaces the values of A on the device: d_vals holds nnz values in input
+ * COO order, of the plan's vector type (double for F64 plans, float for F32
+ * and BF16 plans, rounded to the storage type).  Duplicates are summed in
+ * input order in FP64.  Updates A^T too when it exists.  After this call
+ * the plan runs the stream schedule only, and cbspmv_export still reports
+ * the build-time values. */
+int cbspmv_set_values(cbspmv_plan* plan, const void* d_vals, void* stream);
 
 /* ---- input -------------------------------------------------------------------- */
 /* Reads a Matrix Market coordinate file the way the reference's reader does
