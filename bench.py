@@ -15,6 +15,21 @@ cpu_baseline = the reference's own CPU path (check_results_consistency, its
              sequential FP64 SpMV, compiled from the reference sources into
              oracle/_ref), timed on the host on the same matrix; the oracle's
              restatement of the same loop when oracle/_ref is absent
+
+Other workloads (--workload), one JSON line each:
+  precision  the 2,097,152^2 block-clustered matrix (~50M nnz, values scaled
+             by 10^U(-3,3) per row) with values stored FP64 / FP32 / BF16,
+             FP32 native and compensated accumulation: GB/s of each, FP32/FP64
+             throughput ratio, relative L2 error of each against the FP64
+             oracle
+  autograd   the 524,288^2 power-law matrix (~20M nnz, FP32): forward time,
+             backward time per mode (dx only; dx + dvals fused; dx + dvals
+             direct; stored-order), fwd+bwd / fwd, gradient errors against
+             FP64
+  batched    256 matrices (n log-uniform in [1e3, 2e5], 1-8 FP32 right-hand
+             sides each): autotuned plans (16/32 blocks x 4/9 waves), one
+             persistent launch, eager and hipGraph-replayed; B_alg summed
+             over matrices with x and y counted per right-hand side
 """
 import argparse
 import json
@@ -89,6 +104,199 @@ def cpu_baseline(d, budget_s):
     return y_ref, t, reps, kind, what
 
 
+def timed(fn, steps, warmup, stream):
+    """Mean ms per call of fn over `steps` calls after `warmup` calls,
+    between events on `stream` (the stream fn launches on)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def bench_precision(args):
+    import numpy as np
+    import torch
+
+    import cbspmv
+    from cbspmv import gen
+    from oracle import pyoracle
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    d = gen.config3()
+    m, n, nnz = d["m"], d["n"], int(d["row"].size)
+    log(f"precision matrix {m}x{n}, nnz {nnz}, generated in {time.perf_counter() - t0:.1f}s")
+    threads = min(16, os.cpu_count() or 1)
+    y64 = pyoracle.spmv_csr_omp(m, d["row"], d["col"], d["val"], d["x"], threads)
+    stream = torch.cuda.current_stream()
+    res = {}
+    for dtype, comp in (("f64", False), ("f32", False), ("f32", True), ("bf16", False), ("bf16", True)):
+        plan = cbspmv.Plan(m, n, d["row"], d["col"], d["val"], dtype=dtype, compensated=comp)
+        vdt = plan.vector_dtype
+        x = torch.tensor(d["x"], dtype=vdt, device=dev)
+        y = torch.empty(m, dtype=vdt, device=dev)
+        plan.upload(stream)
+        ms = timed(lambda: plan.spmv(x, y, stream), args.steps, args.warmup, stream)
+        rel = pyoracle.rel_l2(y.double().cpu().numpy(), y64)
+        b_alg = algorithmic_bytes(m, n, nnz, dtype)
+        key = dtype + ("_comp" if comp else "")
+        res[key] = {"us": round(ms * 1e3, 2), "GB_s": round(b_alg / (ms * 1e-3) / 1e9, 1),
+                    "GFLOP_s": round(2 * nnz / (ms * 1e-3) / 1e9, 1), "rel_l2_vs_fp64": rel,
+                    "bytes_alg": b_alg, "stream_slab_bytes": plan.stats()["stream_bytes"]}
+        log(f"{key}: {ms * 1e3:.1f} us, rel {rel:.2e}")
+        plan.close()
+    speed = res["f64"]["us"] / res["f32"]["us"]
+    return {"metric": "FP32-storage SpMV throughput / FP64 on the mixed-precision matrix", "value": round(speed, 3),
+            "unit": "x", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": res["f32"]["us"] / 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded generator, cbspmv/gen.py config3)",
+            "config": {"workload": "clustered_2M_50M_nnz_mixed_precision", "m": m, "n": n, "nnz": nnz},
+            "results": res}
+
+
+def bench_autograd(args):
+    import numpy as np
+    import torch
+
+    import cbspmv
+    from cbspmv import gen
+    from oracle import pyoracle
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    d = gen.config5()
+    m, n, r, c, v = d["m"], d["n"], d["row"], d["col"], d["val"]
+    nnz = int(r.size)
+    log(f"autograd matrix {m}x{n}, nnz {nnz}, generated in {time.perf_counter() - t0:.1f}s")
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(7)
+    g64 = rng.uniform(-1, 1, m)
+    x = torch.tensor(d["x"], dtype=torch.float32, device=dev)
+    g = torch.tensor(g64, dtype=torch.float32, device=dev)
+    threads = min(16, os.cpu_count() or 1)
+    o = np.argsort(c, kind="stable")
+    dx64 = pyoracle.spmv_csr_omp(n, c[o], r[o], v[o], g64, threads)
+    dv64 = g64[r] * d["x"][c]
+    res = {}
+    for mode in ("fused", "direct"):
+        plan = cbspmv.Plan(m, n, r, c, v, dtype="f32", backward=mode)
+        t1 = time.perf_counter()
+        plan.build_transpose()
+        log(f"{mode}: transposed plan built in {time.perf_counter() - t1:.1f}s")
+        plan.upload(stream)
+        y = torch.empty(m, dtype=torch.float32, device=dev)
+        dx = torch.empty(n, dtype=torch.float32, device=dev)
+        dv = torch.empty(nnz, dtype=torch.float32, device=dev)
+        lib = cbspmv.lib()
+        import ctypes
+
+        h, sh = plan.handle, ctypes.c_void_p(stream.cuda_stream)
+        P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        fwd = timed(lambda: plan.spmv(x, y, stream), args.steps, args.warmup, stream)
+        both = timed(lambda: lib.cbspmv_backward(h, P(g), P(x), P(dx), P(dv), sh), args.steps, args.warmup, stream)
+        err_dx = pyoracle.rel_l2(dx.double().cpu().numpy(), dx64)
+        err_dv = pyoracle.rel_l2(dv.double().cpu().numpy(), dv64)
+        only_dx = timed(lambda: lib.cbspmv_backward(h, P(g), P(x), P(dx), ctypes.c_void_p(0), sh), args.steps,
+                        args.warmup, stream)
+        entry = {"fwd_us": round(fwd * 1e3, 2), "bwd_dx_dvals_us": round(both * 1e3, 2),
+                 "bwd_dx_only_us": round(only_dx * 1e3, 2), "ratio_fwd_bwd_over_fwd": round((fwd + both) / fwd, 3),
+                 "ratio_dx_only": round((fwd + only_dx) / fwd, 3), "rel_l2_dx": err_dx, "rel_l2_dvals": err_dv}
+        if mode == "fused":
+            st = torch.empty(plan.stored_entries(), dtype=torch.float32, device=dev)
+            stored = timed(lambda: lib.cbspmv_backward_stored(h, P(g), P(x), P(dx), P(st), sh), args.steps,
+                           args.warmup, stream)
+            entry["bwd_stored_order_us"] = round(stored * 1e3, 2)
+            entry["ratio_stored_order"] = round((fwd + stored) / fwd, 3)
+        res[mode] = entry
+        log(f"{mode}: {entry}")
+        plan.close()
+    best = min(res.values(), key=lambda e: e["ratio_fwd_bwd_over_fwd"])
+    return {"metric": "fwd+bwd / fwd time, differentiable SpMV (dx and dvals in input order)",
+            "value": best["ratio_fwd_bwd_over_fwd"], "unit": "x", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": (best["fwd_us"] + best["bwd_dx_dvals_us"]) / 1e3,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded generator, cbspmv/gen.py config5)",
+            "config": {"workload": "powerlaw_512k_20M_nnz_autograd", "m": m, "n": n, "nnz": nnz}, "results": res}
+
+
+def bench_batched(args):
+    import numpy as np
+    import torch
+
+    import cbspmv
+    from cbspmv import gen
+    from oracle import pyoracle
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    mats = gen.config4(count=args.batch_count)
+    log(f"{len(mats)} matrices, {sum(int(d['row'].size) for d in mats)} nnz, generated in "
+        f"{time.perf_counter() - t0:.1f}s")
+    stream = torch.cuda.current_stream()
+    plans, picks, X, Y = [], {}, [], []
+    t0 = time.perf_counter()
+    for d in mats:
+        if args.no_autotune:
+            p = cbspmv.Plan(d["m"], d["n"], d["row"], d["col"], d["val"], dtype="f32")
+        else:
+            p, _ = cbspmv.autotune(d["m"], d["n"], d["row"], d["col"], d["val"], nrhs=d["nrhs"], dtype="f32")
+            w = int(p.export("stream_tile_shape")[0])
+            bs = (d["m"] + p.stats()["gb_m"] - 1) // max(1, p.stats()["gb_m"])
+            bs = 16 if bs <= 16 else 32
+            picks[f"bs{bs}_w{w}"] = picks.get(f"bs{bs}_w{w}", 0) + 1
+        plans.append(p)
+        X.append(torch.tensor(d["X"], dtype=torch.float32, device=dev))
+        Y.append(torch.empty((d["nrhs"], d["m"]), dtype=torch.float32, device=dev))
+    log(f"plans ready in {time.perf_counter() - t0:.1f}s; autotuner picks {picks}")
+    b = cbspmv.Batch(plans, stream)
+    batched = timed(lambda: b.spmv(X, Y, stream), args.steps, args.warmup, stream)
+    # correctness: every right-hand side against the FP64 oracle
+    worst = 0.0
+    for d, y in zip(mats, Y):
+        ys = y.double().cpu().numpy()
+        for k in range(d["nrhs"]):
+            ref = pyoracle.spmv_seq(d["m"], d["row"], d["col"], d["val"], d["X"][k].astype(np.float64))
+            worst = max(worst, pyoracle.rel_l2(ys[k], ref))
+    gs = torch.cuda.Stream()
+    with torch.cuda.stream(gs):
+        b.capture(X, Y, gs)
+    gs.synchronize()
+    graph = timed(lambda: b.replay(gs), args.steps, args.warmup, gs)
+
+    def eager_loop():
+        for p, x, y in zip(plans, X, Y):
+            for k in range(x.shape[0]):
+                p.spmv(x[k], y[k], stream)
+    loop = timed(eager_loop, max(3, args.steps // 20), 2, stream)
+    b_alg = 0
+    for d in mats:
+        nnz = int(d["row"].size)
+        b_alg += nnz * 8 + 4 * (d["m"] + 1) + d["nrhs"] * 4 * (d["n"] + d["m"])
+    gbs = b_alg / (graph * 1e-3) / 1e9
+    b.close()
+    for p in plans:
+        p.close()
+    return {"metric": "batched SpMV GB/s (sum of B_alg, x and y per right-hand side), hipGraph replay",
+            "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": graph, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded generator, cbspmv/gen.py config4)",
+            "config": {"workload": f"batched_{len(mats)}_matrices_1to8_rhs", "matrices": len(mats),
+                       "nnz": sum(int(d["row"].size) for d in mats), "rhs": sum(d["nrhs"] for d in mats)},
+            "results": {"graph_replay_ms": round(graph, 4), "eager_batched_ms": round(batched, 4),
+                        "per_matrix_launch_loop_ms": round(loop, 4), "bytes_alg": b_alg,
+                        "GB_s_eager": round(b_alg / (batched * 1e-3) / 1e9, 1),
+                        "worst_rel_l2_vs_fp64": worst, "autotune_picks": picks}}
+
+
 def read_traffic(workload_key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -111,6 +319,9 @@ def main():
     ap.add_argument("--item-nnz", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="spmv", choices=["spmv", "precision", "autograd", "batched"])
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (for scaling PMC byte counters)")
     args = ap.parse_args()
@@ -129,6 +340,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 5}), flush=True)
+        return
+
+    if args.workload != "spmv":
+        fn = {"precision": bench_precision, "autograd": bench_autograd, "batched": bench_batched}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
         return
 
     import cbspmv

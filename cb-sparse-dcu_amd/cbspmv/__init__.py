@@ -7,6 +7,7 @@ device memory and streams.
 from ._lib import CbspmvError, LIB_PATH, lib  # noqa: F401
 from .plan import ARRAY_NAMES, Plan, make_opts, read_mtx  # noqa: F401
 from .autograd import CBSparse, cb_spmv as cb_spmv_autograd  # noqa: F401
+from .batch import Batch, autotune  # noqa: F401
 from .reference_api import (  # noqa: F401
     GLOBAL_BLOCK,
     PROGRAM_METRICS,

@@ -8,9 +8,11 @@ computes y = A x only and has no gradients).
 
 The backward runs hand-written kernels only: dx over the transposed CB plan
 and dvals as the SDDMM over the stored pattern (cbspmv_backward).  Values
-are pushed to the device plans (cbspmv_set_values) when the tensor passed as
-`vals` changed since the last call (new storage or an in-place update).
+are pushed to the device plans (cbspmv_set_values) when `vals` is another
+tensor than the one pushed last, or was updated in place since.
 """
+import weakref
+
 import torch
 
 from .plan import Plan
@@ -29,16 +31,19 @@ class CBSparse:
                 self.plan.upload()
         else:
             self.plan.upload()
-        self._synced = None
+        self._synced = (None, -1)   # (weak reference to the tensor, its version)
 
     def sync_values(self, vals):
+        """Pushes vals to the device plans unless this very tensor, at this
+        version, was pushed last (a new tensor may reuse freed storage, so
+        the storage address alone does not identify it)."""
         if vals.dtype != self.dtype or vals.numel() != self.nnz or vals.device.type != "cuda":
             raise ValueError(f"vals must be a CUDA {self.dtype} tensor of {self.nnz} elements")
-        vals = vals.detach().contiguous()
-        key = (vals.data_ptr(), vals._version)
-        if key != self._synced:
-            self.plan.set_values(vals)
-            self._synced = key
+        ref, ver = self._synced
+        if ref is not None and ref() is vals and ver == vals._version:
+            return
+        self.plan.set_values(vals.detach().contiguous())
+        self._synced = (weakref.ref(vals), vals._version)
 
     def close(self):
         self.plan.close()

@@ -30,6 +30,7 @@ _EXPORT_DTYPES = {
     "stream_tile_lines": np.uint32,
     "stream_empty_rows": np.uint32,
     "grad_slot": np.uint32,
+    "stream_value_pos": np.int64,
     "slot_of": np.uint32,
 }
 ARRAY_NAMES = [

@@ -1,0 +1,340 @@
+// Batched persistent launch, its hipGraph capture, and the autotuner (no
+// reference counterpart: the reference runs one matrix per process, with
+// block size and waves fixed at compile time, cb-spmv-dcu/src/macros.h:38-82).
+//
+// A batch is a set of uploaded plans of one value type.  One launch of
+// stream_batch covers every (plan, tile, right-hand side): the work units
+// (plan, tile) are dealt to a grid sized to the resident capacity of the
+// GPU, and each unit runs its tile once per right-hand side.  The per-plan
+// descriptor (stream arguments, vector pointers, partial-sum buffers per
+// right-hand side) lives in device memory; it is re-uploaded only when the
+// vector pointers or RHS counts change.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <random>
+#include <vector>
+
+#include "cb_runtime.hpp"
+#include "kernels.hpp"
+#include "stream.hpp"
+
+namespace cbs {
+constexpr int kBatchMaxRhs = 16;
+constexpr int kMI355XCUs = 256;
+}  // namespace cbs
+
+struct cbspmv_batch {
+    std::vector<cbspmv_plan*> plans;
+    int dtype = CBSPMV_F64;
+    bool comp = false;
+    uint32_t units = 0;
+    int threads = 64;
+    size_t lds = 0;
+    int grid = 8;
+    cbs::DeviceBuf desc, partial, counters;
+    std::vector<uint8_t> host_desc;   // last uploaded descriptor bytes
+    // captured graph and the descriptor it reads
+    cbs::DeviceBuf graph_desc;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace cbs {
+namespace {
+
+template <typename V, typename A>
+void fill_args(const cbspmv_plan& p, StreamArgs<V, A>& a) {
+    const Plan& h = p.host;
+    const DevicePlan& d = p.dev;
+    a = StreamArgs<V, A>{};
+    a.slab = static_cast<const uint8_t*>(d.key.ptr);
+    a.items = static_cast<const StreamItem*>(d.items.ptr);
+    a.item_split = static_cast<const uint32_t*>(d.item_split.ptr);
+    a.item_split_first = static_cast<const uint32_t*>(d.item_split_first.ptr);
+    a.splits = static_cast<const StreamSplit*>(d.splits.ptr);
+    a.tiles = static_cast<const uint32_t*>(d.tiles.ptr);
+    a.tile_lines = static_cast<const uint32_t*>(d.tile_lines.ptr);
+    a.empty_rows = static_cast<const uint32_t*>(d.empty.ptr);
+    a.n_empty_rows = (uint32_t)h.empty_rows.size();
+    a.n_items = (uint32_t)h.items.size();
+    a.n_tiles = (uint32_t)(h.tiles.size() / 2);
+    a.tile_items = (uint32_t)h.o.tile_items;
+    a.tile_slots = (uint32_t)tile_slots(h.o.tile_bytes, h.o.dtype);
+    a.col_bits = h.col_bits;
+    a.m = h.m;
+    a.n = h.n;
+}
+
+// Descriptor bytes for one call (x/y pointer lists and RHS counts).
+template <typename V, typename A>
+int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* nrhs, std::vector<uint8_t>& out) {
+    const size_t np = b.plans.size();
+    std::vector<BatchPlan<V, A>> d(np);
+    A* part = static_cast<A*>(b.partial.ptr);
+    uint32_t* cnt = static_cast<uint32_t*>(b.counters.ptr);
+    uint32_t unit = 0;
+    for (size_t i = 0; i < np; ++i) {
+        const cbspmv_plan& p = *b.plans[i];
+        const Plan& h = p.host;
+        if (nrhs[i] < 0 || nrhs[i] > kBatchMaxRhs) return fail(CBSPMV_ERR_ARG, "nrhs must be in 0..16");
+        if (nrhs[i] > 0 && ((h.m > 0 && !y[i]) || (h.nnz > 0 && !x[i]))) return fail(CBSPMV_ERR_ARG, "null vector");
+        BatchPlan<V, A>& e = d[i];
+        fill_args(p, e.args);
+        e.unit_begin = unit;
+        unit += std::max<uint32_t>(1u, e.args.n_tiles);
+        e.nrhs = (uint32_t)nrhs[i];
+        e.x = static_cast<const A*>(x[i]);
+        e.y = static_cast<A*>(y[i]);
+        e.ldx = (uint64_t)h.n;
+        e.ldy = (uint64_t)h.m;
+        const size_t parts = b.comp ? 2 : 1;   // (hi, lo) per row piece
+        e.partial_stride = (uint64_t)h.partial_slots * parts;
+        e.counter_stride = h.splits.size();
+        e.partial = part;
+        e.counters = cnt;
+        part += (size_t)h.partial_slots * parts * kBatchMaxRhs;
+        cnt += h.splits.size() * kBatchMaxRhs;
+    }
+    out.resize(np * sizeof(BatchPlan<V, A>));
+    if (np) std::memcpy(out.data(), d.data(), out.size());
+    return CBSPMV_OK;
+}
+
+int build_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* nrhs, std::vector<uint8_t>& out) {
+    if (b.dtype == CBSPMV_F64) return make_desc<double, double>(b, x, y, nrhs, out);
+    if (b.dtype == CBSPMV_F32) return make_desc<float, float>(b, x, y, nrhs, out);
+    return make_desc<dev::bf16_t, float>(b, x, y, nrhs, out);
+}
+
+int upload_desc(DeviceBuf& buf, const std::vector<uint8_t>& bytes, hipStream_t s) {
+    if (buf.bytes < bytes.size()) {
+        const int rc = upload_bytes(buf, nullptr, bytes.size(), s);
+        if (rc) return rc;
+    }
+    if (!bytes.empty()) {
+        CBS_HIP(hipMemcpyAsync(buf.ptr, bytes.data(), bytes.size(), hipMemcpyHostToDevice, s));
+        CBS_HIP(hipStreamSynchronize(s));   // the host bytes may change after return
+    }
+    return CBSPMV_OK;
+}
+
+int launch(cbspmv_batch& b, const void* desc, hipStream_t s) {
+    if (b.plans.empty()) return CBSPMV_OK;
+    hipError_t e;
+    const uint32_t np = (uint32_t)b.plans.size();
+    if (b.dtype == CBSPMV_F64)
+        e = launch_batch(static_cast<const BatchPlan<double, double>*>(desc), np, b.units, b.grid, b.threads, b.lds,
+                         false, s);
+    else if (b.dtype == CBSPMV_F32)
+        e = launch_batch(static_cast<const BatchPlan<float, float>*>(desc), np, b.units, b.grid, b.threads, b.lds,
+                         b.comp, s);
+    else
+        e = launch_batch(static_cast<const BatchPlan<dev::bf16_t, float>*>(desc), np, b.units, b.grid, b.threads,
+                         b.lds, b.comp, s);
+    return e == hipSuccess ? CBSPMV_OK : hip_fail(e, "batched launch");
+}
+
+void drop_graph(cbspmv_batch& b) {
+    if (b.exec) (void)hipGraphExecDestroy(b.exec);
+    if (b.graph) (void)hipGraphDestroy(b.graph);
+    b.exec = nullptr;
+    b.graph = nullptr;
+}
+
+}  // namespace
+}  // namespace cbs
+
+using cbs::fail;
+
+extern "C" {
+
+int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** batch, void* stream) {
+    if (!batch || count < 0 || (count > 0 && !plans)) return fail(CBSPMV_ERR_ARG, "null argument");
+    *batch = nullptr;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    cbspmv_batch* b = new (std::nothrow) cbspmv_batch();
+    if (!b) return fail(CBSPMV_ERR_NOMEM, "out of host memory");
+    size_t partial_elems = 0, counter_elems = 0;
+    int max_items = 1, max_slots = 0;
+    for (int i = 0; i < count; ++i) {
+        cbspmv_plan* p = plans[i];
+        if (!p) { delete b; return fail(CBSPMV_ERR_ARG, "null plan in batch"); }
+        const cbs::Plan& h = p->host;
+        if (i == 0) { b->dtype = h.o.dtype; b->comp = h.o.accum == CBSPMV_ACC_COMPENSATED; }
+        if (h.o.dtype != b->dtype || (h.o.accum == CBSPMV_ACC_COMPENSATED) != b->comp) {
+            delete b;
+            return fail(CBSPMV_ERR_ARG, "plans of one batch must share value type and accumulation");
+        }
+        const int rc = cbs::upload_plan(*p, s);
+        if (rc) { delete b; return rc; }
+        b->plans.push_back(p);
+        b->units += std::max<uint32_t>(1u, (uint32_t)(h.tiles.size() / 2));
+        partial_elems += (size_t)h.partial_slots * cbs::kBatchMaxRhs * 2;   // room for (hi, lo)
+        counter_elems += h.splits.size() * cbs::kBatchMaxRhs;
+        max_items = std::max(max_items, h.o.tile_items);
+        max_slots = std::max(max_slots, cbs::tile_slots(h.o.tile_bytes, h.o.dtype));
+    }
+    const size_t eb = b->dtype == CBSPMV_F64 ? 8 : 4;
+    b->threads = 64 * max_items;
+    b->lds = (size_t)max_slots * eb;
+    // a persistent grid: as many workgroups as stay resident at once
+    int per_cu = cbs::batch_blocks_per_cu(b->dtype, b->comp, b->threads, b->lds);
+    if (per_cu <= 0) {   // occupancy query unavailable: bound by LDS and 32 waves per CU
+        const int by_lds = b->lds ? (int)std::max<size_t>(1, (160 * 1024) / b->lds) : 8;
+        per_cu = std::max(1, std::min(by_lds, 32 / max_items));
+    }
+    const uint32_t cap = (uint32_t)(cbs::kMI355XCUs * per_cu);
+    b->grid = (int)((std::min<uint32_t>(std::max<uint32_t>(b->units, 1u), cap) + 7u) / 8u * 8u);
+    int rc;
+    if ((rc = cbs::upload_bytes(b->partial, nullptr, std::max<size_t>(partial_elems, 1) * eb, s)) ||
+        (rc = cbs::upload_bytes(b->counters, nullptr, std::max<size_t>(counter_elems, 1) * 4, s))) {
+        cbspmv_batch_destroy(b);
+        return rc;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        cbspmv_batch_destroy(b);
+        return fail(CBSPMV_ERR_HIP, "batch create: stream synchronize");
+    }
+    *batch = b;
+    return CBSPMV_OK;
+}
+
+int cbspmv_batch_spmv(cbspmv_batch* batch, const void* const* d_x, void* const* d_y, const int* nrhs,
+                      void* stream) {
+    if (!batch) return fail(CBSPMV_ERR_ARG, "null batch");
+    if (batch->plans.empty()) return CBSPMV_OK;
+    if (!d_x || !d_y || !nrhs) return fail(CBSPMV_ERR_ARG, "null argument");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    std::vector<uint8_t> bytes;
+    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, bytes);
+    if (rc) return rc;
+    if (bytes != batch->host_desc) {
+        if ((rc = cbs::upload_desc(batch->desc, bytes, s))) return rc;
+        batch->host_desc.swap(bytes);
+    }
+    return cbs::launch(*batch, batch->desc.ptr, s);
+}
+
+int cbspmv_batch_capture(cbspmv_batch* batch, const void* const* d_x, void* const* d_y, const int* nrhs,
+                         void* stream) {
+    if (!batch) return fail(CBSPMV_ERR_ARG, "null batch");
+    if (!stream) return fail(CBSPMV_ERR_ARG, "graph capture needs a created (non-null) stream");
+    if (!d_x || !d_y || !nrhs) return fail(CBSPMV_ERR_ARG, "null argument");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    std::vector<uint8_t> bytes;
+    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, bytes);
+    if (rc) return rc;
+    if ((rc = cbs::upload_desc(batch->graph_desc, bytes, s))) return rc;
+    cbs::drop_graph(*batch);
+    CBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = cbs::launch(*batch, batch->graph_desc.ptr, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return cbs::hip_fail(e, "hipStreamEndCapture");
+    batch->graph = g;
+    CBS_HIP(hipGraphInstantiate(&batch->exec, g, nullptr, nullptr, 0));
+    CBS_HIP(hipGraphLaunch(batch->exec, s));
+    return CBSPMV_OK;
+}
+
+int cbspmv_batch_replay(cbspmv_batch* batch, void* stream) {
+    if (!batch) return fail(CBSPMV_ERR_ARG, "null batch");
+    if (!batch->exec) return fail(CBSPMV_ERR_STATE, "nothing captured (call cbspmv_batch_capture first)");
+    CBS_HIP(hipGraphLaunch(batch->exec, static_cast<hipStream_t>(stream)));
+    return CBSPMV_OK;
+}
+
+void cbspmv_batch_destroy(cbspmv_batch* batch) {
+    if (!batch) return;
+    cbs::drop_graph(*batch);
+    for (cbs::DeviceBuf* d : {&batch->desc, &batch->partial, &batch->counters, &batch->graph_desc}) cbs::release(*d);
+    delete batch;
+}
+
+int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const double* val,
+                    const cbspmv_opts* base, const int* candidates, int ncandidates, int nrhs,
+                    cbspmv_plan** best, double* best_ms, void* stream) {
+    if (!best) return fail(CBSPMV_ERR_ARG, "null output");
+    *best = nullptr;
+    if (nrhs < 1 || nrhs > cbs::kBatchMaxRhs) return fail(CBSPMV_ERR_ARG, "nrhs must be in 1..16");
+    static const int kDefault[] = {16, 4, 16, 9, 32, 4, 32, 9};
+    if (!candidates) { candidates = kDefault; ncandidates = 4; }
+    if (ncandidates < 1) return fail(CBSPMV_ERR_ARG, "no candidates");
+    cbspmv_opts o;
+    if (base) o = *base;
+    else cbspmv_default_opts(&o);
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t eb = o.value_dtype == CBSPMV_F64 ? 8 : 4;
+    // one x and y block for all candidates, filled with a fixed pattern
+    void* dx = nullptr;
+    void* dy = nullptr;
+    CBS_HIP(hipMalloc(&dx, std::max<size_t>(1, (size_t)n * nrhs * eb)));
+    if (hipMalloc(&dy, std::max<size_t>(1, (size_t)m * nrhs * eb)) != hipSuccess) {
+        (void)hipFree(dx);
+        return fail(CBSPMV_ERR_NOMEM, "autotune: hipMalloc");
+    }
+    {
+        std::vector<uint8_t> hx((size_t)n * nrhs * eb);
+        std::mt19937 gen(12345);
+        std::uniform_real_distribution<double> u(-1.0, 1.0);
+        for (size_t i = 0; i < (size_t)n * nrhs; ++i) {
+            if (eb == 8) { const double v = u(gen); std::memcpy(&hx[i * 8], &v, 8); }
+            else { const float v = (float)u(gen); std::memcpy(&hx[i * 4], &v, 4); }
+        }
+        if (!hx.empty()) (void)hipMemcpy(dx, hx.data(), hx.size(), hipMemcpyHostToDevice);
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    int rc = CBSPMV_OK;
+    double best_t = 1e300;
+    for (int c = 0; c < ncandidates && rc == CBSPMV_OK; ++c) {
+        cbspmv_opts oc = o;
+        oc.block_size = candidates[2 * c];
+        oc.waves_per_block = candidates[2 * c + 1];
+        oc.stream_tile_waves = candidates[2 * c + 1];
+        cbspmv_plan* p = nullptr;
+        rc = cbspmv_build(m, n, nnz, row, col, val, &oc, &p);
+        if (rc) break;
+        if ((rc = cbs::upload_plan(*p, s))) { cbspmv_destroy(p); break; }
+        auto run = [&]() {
+            for (int r = 0; r < nrhs && rc == CBSPMV_OK; ++r)
+                rc = cbs::run_spmv(*p, static_cast<uint8_t*>(dx) + (size_t)r * n * eb,
+                                   static_cast<uint8_t*>(dy) + (size_t)r * m * eb, s);
+        };
+        for (int w = 0; w < 3; ++w) run();
+        constexpr int kReps = 20;
+        (void)hipEventRecord(e0, s);
+        for (int k = 0; k < kReps; ++k) run();
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double t = ms / kReps;
+        if (rc == CBSPMV_OK && t < best_t) {
+            if (*best) cbspmv_destroy(*best);
+            *best = p;
+            best_t = t;
+        } else {
+            cbspmv_destroy(p);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    if (rc) {
+        if (*best) cbspmv_destroy(*best);
+        *best = nullptr;
+        return rc;
+    }
+    if (best_ms) *best_ms = best_t;
+    return CBSPMV_OK;
+}
+
+}  // extern "C"

@@ -592,6 +592,14 @@ long export_array(const Plan& p, const std::string& s, void* dst, size_t cap, in
         return copy_out(v, dst, cap);
     }
     if (s == "stream_col_bits") return copy_out(std::vector<int>{p.col_bits}, dst, cap);
+    if (s == "stream_value_pos") {   // byte offset in stream_slab of each entry's value
+        std::vector<int64_t> v;
+        v.reserve(p.s_key.size());
+        for (size_t i = 0; i < p.items.size(); ++i)
+            for (uint32_t e = p.items[i].e_begin; e < p.items[i].e_end; ++e)
+                v.push_back((int64_t)stream_value_offset(p, (uint32_t)i, e));
+        return copy_out(v, dst, cap);
+    }
     if (s == "policy") {
         // the reference's dispatch decision (cb-spmv-dcu.hpp:1428-1450):
         // unit_values, use_pattern, use_unrolled, use_small_direct,

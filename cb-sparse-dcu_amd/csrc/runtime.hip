@@ -93,7 +93,9 @@ int upload_plan(cbspmv_plan& p, hipStream_t s) {
     if ((rc = upload_vec(d.item_split, h.item_split, s))) return rc;
     if ((rc = upload_vec(d.item_split_first, h.item_split_first, s))) return rc;
     if ((rc = upload_vec(d.splits, h.splits, s))) return rc;
-    if ((rc = upload_bytes(d.partial, nullptr, (size_t)h.partial_slots * vec_elem_bytes(h), s))) return rc;
+    // compensated plans keep (hi, lo) per row piece
+    const size_t parts = h.o.accum == CBSPMV_ACC_COMPENSATED && h.o.dtype != CBSPMV_F64 ? 2 : 1;
+    if ((rc = upload_bytes(d.partial, nullptr, (size_t)h.partial_slots * parts * vec_elem_bytes(h), s))) return rc;
     if ((rc = upload_bytes(d.counters, nullptr, h.splits.size() * 4, s))) return rc;
     if ((rc = upload_vec(d.empty, h.empty_rows, s))) return rc;
     if ((rc = upload_vec(d.tiles, h.tiles, s))) return rc;

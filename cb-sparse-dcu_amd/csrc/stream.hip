@@ -110,26 +110,17 @@ __device__ __forceinline__ double wave_shr1(double v) { return dpp<kWaveShr1, 0x
 __device__ __forceinline__ float wave_shr1(float v) { return dpp<kWaveShr1, 0xf>(v); }
 __device__ __forceinline__ f2 wave_shr1(f2 v) { return f2{dpp<kWaveShr1, 0xf>(v.hi), dpp<kWaveShr1, 0xf>(v.lo)}; }
 
+// One tile of the stream: stage its hot x lines in LDS, then each of its
+// waves streams one item.  Every thread of the workgroup calls this (it
+// holds a workgroup barrier); waves without an item only stage.
 template <typename V, typename A, bool COMP, int EPL, bool SDDMM>
-__global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A> a) {
+__device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t tile, A* xs) {
     using Acc = typename std::conditional<COMP, f2, A>::type;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    A* xs = reinterpret_cast<A*>(lds_raw);   // tile_slots elements (dynamic LDS)
     const uint32_t SLOTS = a.tile_slots;
     const uint32_t kZero = SLOTS - 1;
     const uint32_t nthreads = blockDim.x;
-
     const int lane = lane_id();
     const int wave = (int)(threadIdx.x >> 6);
-    // XCD-aware order: the hardware deals workgroups round-robin over the 8
-    // XCDs, so map them to contiguous tile ranges per XCD (x windows of
-    // neighbouring tiles then share that XCD's L2).
-    uint32_t tile = blockIdx.x;
-    if ((gridDim.x & 7u) == 0) tile = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-
-    for (uint32_t t = blockIdx.x * nthreads + threadIdx.x; t < a.n_empty_rows; t += gridDim.x * nthreads)
-        a.y[a.empty_rows[t]] = A(0);
-    if (tile >= a.n_tiles) return;
 
     // ---- stage this tile's hot x lines in LDS, PER slots per thread per
     // pass with all line-id loads, then all x loads, issued back to back (16
@@ -162,6 +153,7 @@ __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A
     }
     __syncthreads();
 
+    if ((uint32_t)wave >= a.tile_items) return;
     const uint32_t item = tile * a.tile_items + wave;
     if (item >= a.n_items) return;
     const StreamItem it = a.items[item];
@@ -174,16 +166,26 @@ __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A
     // A finished run (row offset `off`) goes to y, or to the partial buffer
     // when it is a piece of a cut row (the item's first run, offset 0, or its
     // last run).
-    auto emit = [&](bool go, int off, A v, bool final_run) {
+    // Pieces keep their full accumulator (hi and lo of a compensated sum).
+    constexpr int kParts = COMP ? 2 : 1;
+    auto put_partial = [&](uint32_t slot, Acc v) {
+        if constexpr (COMP) {
+            a.partial[2 * slot] = v.hi;
+            a.partial[2 * slot + 1] = v.lo;
+        } else {
+            a.partial[slot] = v;
+        }
+    };
+    auto emit = [&](bool go, int off, Acc v, bool final_run) {
         if (!go) return;
         if (it.first_partial && off == 0) {
             const StreamSplit sp = a.splits[split_first];
-            a.partial[sp.partial_base + (item - sp.first_item)] = v;
+            put_partial(sp.partial_base + (item - sp.first_item), v);
         } else if (final_run && it.last_partial) {
             const StreamSplit sp = a.splits[split_last];
-            a.partial[sp.partial_base + (item - sp.first_item)] = v;
+            put_partial(sp.partial_base + (item - sp.first_item), v);
         } else {
-            a.y[it.row_base + (uint32_t)off] = v;
+            a.y[it.row_base + (uint32_t)off] = finish(v);
         }
     };
     auto gather = [&](const uint32_t* kk, A* xx) {
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A
             if (o0 == carry_off) {
                 if (lane == 0) s[0] += carry;
             } else {
-                emit(lane == 0, carry_off, finish(carry), false);
+                emit(lane == 0, carry_off, carry, false);
             }
         }
         // runs inside the lane: s[j] becomes the partial sum of the run
@@ -264,10 +266,10 @@ __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A
         for (int j = 0; j < EPL - 1; ++j) {
             Acc v = s[j];
             if (first && prev_off == off[0]) v += prev_T;
-            emit(valid && ends[j], off[j], finish(v), false);
+            emit(valid && ends[j], off[j], v, false);
             first = first && !ends[j];
         }
-        emit(valid && lane < L && next_off != off[EPL - 1], off[EPL - 1], finish(T), false);
+        emit(valid && lane < L && next_off != off[EPL - 1], off[EPL - 1], T, false);
         carry = read_lane(T, L);
         carry_off = __builtin_amdgcn_readlane(off[EPL - 1], L);
         if (more) {
@@ -275,7 +277,7 @@ __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A
             gather(cur.k, xc);
         }
     }
-    emit(lane == 0, carry_off, finish(carry), true);
+    emit(lane == 0, carry_off, carry, true);
 
     // ---- pieces of a cut row: the last piece to arrive sums them in order
     for (int which = 0; which < 2; ++which) {
@@ -289,11 +291,69 @@ __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A
         if (ticket == sp.pieces - 1) {
             __threadfence();
             if (lane == 0) {
-                A sum = A(0);
-                for (uint32_t q = 0; q < sp.pieces; ++q) sum += load_partial(a.partial + sp.partial_base + q);
-                a.y[sp.row] = sum;
+                Acc sum{};
+                for (uint32_t q = 0; q < sp.pieces; ++q) {
+                    const uint32_t slot = kParts * (sp.partial_base + q);
+                    if constexpr (COMP) sum += f2{load_partial(a.partial + slot), load_partial(a.partial + slot + 1)};
+                    else sum += load_partial(a.partial + slot);
+                }
+                a.y[sp.row] = finish(sum);
                 atomicExch(a.counters + sid, 0u);
             }
+        }
+    }
+}
+
+template <typename V, typename A, bool COMP, int EPL, bool SDDMM>
+__global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    A* xs = reinterpret_cast<A*>(lds_raw);   // tile_slots elements (dynamic LDS)
+    // XCD-aware order: the hardware deals workgroups round-robin over the 8
+    // XCDs, so map them to contiguous tile ranges per XCD (x windows of
+    // neighbouring tiles then share that XCD's L2).
+    uint32_t tile = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) tile = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const uint32_t nthreads = blockDim.x;
+    for (uint32_t t = blockIdx.x * nthreads + threadIdx.x; t < a.n_empty_rows; t += gridDim.x * nthreads)
+        a.y[a.empty_rows[t]] = A(0);
+    if (tile >= a.n_tiles) return;
+    stream_tile<V, A, COMP, EPL, SDDMM>(a, tile, xs);
+}
+
+// Batched persistent launch: work units are (plan, tile) pairs, dealt to
+// workgroups per XCD in contiguous ranges; a unit runs its tile for every
+// right-hand side of its plan in turn (the tile's slab is then re-read from
+// L2).  Unit 0 of a plan also zeroes the plan's empty rows.
+template <typename V, typename A, bool COMP, int EPL>
+__global__ __launch_bounds__(kMaxStreamThreads) void stream_batch(const BatchPlan<V, A>* plans, uint32_t nplans,
+                                                                  uint32_t units) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    A* xs = reinterpret_cast<A*>(lds_raw);
+    const uint32_t groups = gridDim.x >> 3;   // workgroups per XCD (grid is a multiple of 8)
+    const uint32_t xcd = blockIdx.x & 7u, rank = blockIdx.x >> 3;
+    const uint32_t per = (units + 7u) >> 3;
+    for (uint32_t v = rank; v < per; v += groups) {
+        const uint32_t u = xcd * per + v;
+        if (u >= units) break;
+        // plan of unit u: last plan with unit_begin <= u (binary search)
+        uint32_t lo = 0, hi = nplans;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (plans[mid].unit_begin <= u) lo = mid;
+            else hi = mid;
+        }
+        const BatchPlan<V, A>& bp = plans[lo];
+        const uint32_t tile = u - bp.unit_begin;
+        for (uint32_t r = 0; r < bp.nrhs; ++r) {
+            StreamArgs<V, A> a = bp.args;
+            a.x = bp.x + (size_t)r * bp.ldx;
+            a.y = bp.y + (size_t)r * bp.ldy;
+            a.partial = bp.partial + (size_t)r * bp.partial_stride;
+            a.counters = bp.counters + (size_t)r * bp.counter_stride;
+            if (tile == 0)
+                for (uint32_t t = threadIdx.x; t < a.n_empty_rows; t += blockDim.x) a.y[a.empty_rows[t]] = A(0);
+            if (tile < a.n_tiles) stream_tile<V, A, COMP, EPL, false>(a, tile, xs);
+            __syncthreads();   // LDS is restaged by the next tile
         }
     }
 }
@@ -319,6 +379,51 @@ template <typename V, typename A>
 hipError_t launch_stream_any(const StreamArgs<V, A>& a, int grid, bool comp, bool sddmm, hipStream_t s) {
     if (comp) return sddmm ? launch_stream_t<V, A, true, true>(a, grid, s) : launch_stream_t<V, A, true, false>(a, grid, s);
     return sddmm ? launch_stream_t<V, A, false, true>(a, grid, s) : launch_stream_t<V, A, false, false>(a, grid, s);
+}
+
+template <typename V, typename A, bool COMP>
+hipError_t launch_batch_t(const BatchPlan<V, A>* plans, uint32_t nplans, uint32_t units, int grid, int threads,
+                          size_t lds, hipStream_t s) {
+    constexpr int EPL = sizeof(V) >= 8 ? 4 : 8;
+    auto kern = dev::stream_batch<V, A, COMP, EPL>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        const hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, s, plans, nplans, units);
+    return hipGetLastError();
+}
+
+int batch_blocks_per_cu(int dtype, bool comp, int threads, size_t lds) {
+    int n = 0;
+    hipError_t e;
+    if (dtype == CBSPMV_F64)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::stream_batch<double, double, false, 4>, threads, lds);
+    else if (dtype == CBSPMV_F32)
+        e = comp ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::stream_batch<float, float, true, 8>, threads, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::stream_batch<float, float, false, 8>, threads, lds);
+    else
+        e = comp ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::stream_batch<dev::bf16_t, float, true, 8>, threads, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::stream_batch<dev::bf16_t, float, false, 8>, threads, lds);
+    return e == hipSuccess ? n : 0;
+}
+
+hipError_t launch_batch(const BatchPlan<double, double>* p, uint32_t np, uint32_t units, int grid, int threads,
+                        size_t lds, bool comp, hipStream_t s) {
+    (void)comp;
+    return launch_batch_t<double, double, false>(p, np, units, grid, threads, lds, s);
+}
+hipError_t launch_batch(const BatchPlan<float, float>* p, uint32_t np, uint32_t units, int grid, int threads,
+                        size_t lds, bool comp, hipStream_t s) {
+    return comp ? launch_batch_t<float, float, true>(p, np, units, grid, threads, lds, s)
+                : launch_batch_t<float, float, false>(p, np, units, grid, threads, lds, s);
+}
+hipError_t launch_batch(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t units, int grid, int threads,
+                        size_t lds, bool comp, hipStream_t s) {
+    return comp ? launch_batch_t<dev::bf16_t, float, true>(p, np, units, grid, threads, lds, s)
+                : launch_batch_t<dev::bf16_t, float, false>(p, np, units, grid, threads, lds, s);
 }
 
 hipError_t launch_stream(const StreamArgs<double, double>& a, int grid, bool sddmm, hipStream_t s) {

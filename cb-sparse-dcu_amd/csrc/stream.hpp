@@ -39,6 +39,31 @@ struct StreamArgs {
     A* dvals;
 };
 
+// One plan of a batched launch: its stream arguments (x, y, partial and
+// counters are set per right-hand side from the fields below) and the first
+// work unit (tile) it owns in the batch.
+template <typename V, typename A>
+struct BatchPlan {
+    StreamArgs<V, A> args;
+    uint32_t unit_begin;
+    uint32_t nrhs;
+    const A* x;               // column-major n x nrhs
+    A* y;                     // column-major m x nrhs
+    uint64_t ldx, ldy;
+    A* partial;               // nrhs x partial_stride
+    uint32_t* counters;       // nrhs x counter_stride (zero between launches)
+    uint64_t partial_stride, counter_stride;
+};
+
+// Resident workgroups per CU of the batched kernel for this shape (0 = unknown).
+int batch_blocks_per_cu(int dtype, bool comp, int threads, size_t lds);
+hipError_t launch_batch(const BatchPlan<double, double>* p, uint32_t np, uint32_t units, int grid, int threads,
+                        size_t lds, bool comp, hipStream_t s);
+hipError_t launch_batch(const BatchPlan<float, float>* p, uint32_t np, uint32_t units, int grid, int threads,
+                        size_t lds, bool comp, hipStream_t s);
+hipError_t launch_batch(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t units, int grid, int threads,
+                        size_t lds, bool comp, hipStream_t s);
+
 // sddmm: also write dvals (a.xrow, a.dvals set)
 hipError_t launch_stream(const StreamArgs<double, double>& a, int grid, bool sddmm, hipStream_t s);
 hipError_t launch_stream(const StreamArgs<float, float>& a, int grid, bool comp, bool sddmm, hipStream_t s);

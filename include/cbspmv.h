@@ -186,6 +186,38 @@ int cbspmv_backward_stored(cbspmv_plan* plan, const void* d_g, const void* d_x,
  * the build-time values. */
 int cbspmv_set_values(cbspmv_plan* plan, const void* d_vals, void* stream);
 
+/* ---- batching (no reference counterpart; extension) -------------------------- */
+typedef struct cbspmv_batch cbspmv_batch;
+/* Groups `count` built plans (borrowed; they must outlive the batch) of one
+ * value type and accumulation into a batch: uploads every plan and
+ * allocates partial-sum buffers for up to 16 right-hand sides per plan. */
+int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** batch, void* stream);
+/* Y_i = A_i X_i for every matrix i, with nrhs[i] (0..16) right-hand sides
+ * each: d_x[i] is n_i x nrhs[i] (column-major, leading dim n_i), d_y[i] is
+ * m_i x nrhs[i], of the plans' vector type.  One persistent launch.  The
+ * pointer table is uploaded (with a stream synchronisation) only when the
+ * pointers or counts differ from the previous call. */
+int cbspmv_batch_spmv(cbspmv_batch* batch, const void* const* d_x, void* const* d_y,
+                      const int* nrhs, void* stream);
+/* Captures that launch for these pointers into a hipGraph (stream must be a
+ * created stream, not NULL), instantiates it and launches it once;
+ * cbspmv_batch_replay() launches the instantiated graph again.  The graph
+ * holds no memset nodes. */
+int cbspmv_batch_capture(cbspmv_batch* batch, const void* const* d_x, void* const* d_y,
+                         const int* nrhs, void* stream);
+int cbspmv_batch_replay(cbspmv_batch* batch, void* stream);
+void cbspmv_batch_destroy(cbspmv_batch* batch);
+
+/* Autotuner: builds the candidate plans for one COO matrix, times nrhs
+ * SpMVs of each on the device (3 warm-up, mean of 20) and returns the
+ * fastest, destroying the others.  `candidates` lists (block_size, waves)
+ * pairs; waves sets both the blocked schedule's waves per workgroup and the
+ * stream schedule's tile waves.  NULL = {16,4},{16,9},{32,4},{32,9}.
+ * Allocates device memory for its test vectors. */
+int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const double* val,
+                    const cbspmv_opts* base, const int* candidates, int ncandidates,
+                    int nrhs, cbspmv_plan** best, double* best_ms, void* stream);
+
 /* ---- input -------------------------------------------------------------------- */
 /* Reads a Matrix Market coordinate file the way the reference's reader does
  * (cb-spmv-dcu/src/io/mmio_highlevel.h:62-403, read_matrix_coo_data): every

@@ -128,3 +128,31 @@ def test_transpose_requires_build():
     with pytest.raises(CbspmvError):
         p.export("transpose.gb_row_idx")
     p.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32", "bf16"])
+def test_value_positions_address_the_slab(dtype):
+    """The byte offsets cbspmv_set_values writes to (stream_value_offset)
+    hold each entry's value in the device slab."""
+    m, n, r, c, v = CASES["dense_blocks"]
+    p = Plan(m, n, r, c, v, dtype=dtype, item_nnz=700)
+    pos = p.export("stream_value_pos")
+    slab = p.export("stream_slab")
+    val = p.export("stream_val")
+    vb = {"f64": 8, "f32": 4, "bf16": 2}[dtype]
+    assert pos.size == val.size and np.all(pos % vb == 0) and pos.max() + vb <= slab.size
+    assert np.unique(pos).size == pos.size
+    raw = slab[(pos[:, None] + np.arange(vb)).ravel()].reshape(-1, vb).copy()
+    if dtype == "f64":
+        got = raw.view(np.float64).ravel()
+        assert np.array_equal(got, val)
+    elif dtype == "f32":
+        got = raw.view(np.float32).ravel()
+        assert np.array_equal(got, val.astype(np.float32))
+    else:
+        h = raw.view(np.uint16).ravel().astype(np.uint32) << 16
+        got = h.view(np.float32)
+        f = val.astype(np.float32).view(np.uint32).astype(np.uint64)
+        want = (((f + 0x7FFF + ((f >> 16) & 1)) >> 16).astype(np.uint32) << 16).view(np.float32)
+        assert np.array_equal(got, want)
+    p.close()
