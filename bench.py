@@ -104,6 +104,54 @@ def cpu_baseline(d, budget_s):
     return y_ref, t, reps, kind, what
 
 
+VENDOR_ALGS = {"csr_adaptive": 2, "csr_rowsplit": 3, "csr_lrb": 7, "csr_nnzsplit": 8}
+
+
+def vendor_baseline(d, dtype, reps, ours_ms):
+    """rocSPARSE CSR SpMV (libvendor_csr.so) on the same matrix and GPU, each
+    algorithm timed like ours; the comparator the reference's hipSPARSE
+    baseline programs stand for."""
+    import ctypes
+
+    import numpy as np
+
+    from oracle import pyoracle
+
+    path = os.path.join(ROOT, "cb-sparse-dcu_amd", "libvendor_csr.so")
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path)
+    ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+    L.vendor_csr_spmv.restype = ctypes.c_int
+    L.vendor_csr_spmv.argtypes = [ctypes.c_int] * 3 + [ip, ip, dp, dp, dp] + [ctypes.c_int] * 4 + [dp]
+    m, n, r, c, v, x = d["m"], d["n"], d["row"], d["col"], d["val"], d["x"]
+    ptr = np.zeros(m + 1, np.int32)
+    np.cumsum(np.bincount(r, minlength=m), out=ptr[1:])
+    col = np.ascontiguousarray(c, np.int32)
+    val = np.ascontiguousarray(v, np.float64)
+    xx = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(m)
+    y_ref = pyoracle.spmv_csr_omp(m, r, c, v, x, min(16, os.cpu_count() or 1))
+    b_alg = algorithmic_bytes(m, n, int(r.size), dtype)
+    out = {}
+    for name, alg in VENDOR_ALGS.items():
+        ms = ctypes.c_double()
+        rc = L.vendor_csr_spmv(m, n, int(r.size), ptr.ctypes.data_as(ip), col.ctypes.data_as(ip),
+                               val.ctypes.data_as(dp), xx.ctypes.data_as(dp), y.ctypes.data_as(dp), alg,
+                               int(dtype != "f64"), 20, reps, ctypes.byref(ms))
+        if rc != 0:
+            out[name] = {"status": rc}
+            continue
+        out[name] = {"us": round(ms.value * 1e3, 2), "GB_s": round(b_alg / (ms.value * 1e-3) / 1e9, 1),
+                     "rel_l2_vs_fp64": pyoracle.rel_l2(y, y_ref)}
+        log(f"rocSPARSE {name}: {ms.value * 1e3:.1f} us")
+    ok = {k: e for k, e in out.items() if "us" in e}
+    best = min(ok, key=lambda k: ok[k]["us"]) if ok else None
+    return {"library": "rocSPARSE (ROCm) CSR rocsparse_spmv, same matrix and GPU", "algorithms": out,
+            "best": best, "best_us": ok[best]["us"] if best else None,
+            "cbspmv_speedup_vs_best": round(ok[best]["us"] / (ours_ms * 1e3), 3) if best else None}
+
+
 def timed(fn, steps, warmup, stream):
     """Mean ms per call of fn over `steps` calls after `warmup` calls,
     between events on `stream` (the stream fn launches on)."""
@@ -319,6 +367,7 @@ def main():
     ap.add_argument("--item-nnz", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE comparator")
     ap.add_argument("--workload", default="spmv", choices=["spmv", "precision", "autograd", "batched"])
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--no-autotune", action="store_true")
@@ -425,6 +474,7 @@ def main():
                      "bytes_per_launch_algorithmic": b_alg,
                      "stream_slab_bytes": st["stream_bytes"], "canonical_slab_bytes": st["slab_bytes"]},
         "cpu_baseline": cpu,
+        "vendor_baseline": None if args.no_vendor else vendor_baseline(d, args.dtype, 200, ms),
     }
     print(json.dumps(out), flush=True)
     plan.close()

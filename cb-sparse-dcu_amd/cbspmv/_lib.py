@@ -28,7 +28,7 @@ BUILDER_GATHER, BUILDER_COMP = 0, 1
 F64, F32, BF16 = 0, 1, 2
 ACC_NATIVE, ACC_COMPENSATED = 0, 1
 SCHED_AUTO, SCHED_BLOCKED, SCHED_STREAM = 0, 1, 2
-BACKWARD_FUSED, BACKWARD_DIRECT = 0, 1
+BACKWARD_DIRECT, BACKWARD_FUSED = 0, 1
 
 
 class CbspmvError(RuntimeError):

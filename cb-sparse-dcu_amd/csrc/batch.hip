@@ -169,6 +169,10 @@ int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** bat
             delete b;
             return fail(CBSPMV_ERR_ARG, "plans of one batch must share value type and accumulation");
         }
+        if (h.o.tile_items > cbs::kMaxBatchTileItems) {
+            delete b;
+            return fail(CBSPMV_ERR_UNSUPPORTED, "plans of a batch need stream tile waves <= 10");
+        }
         const int rc = cbs::upload_plan(*p, s);
         if (rc) { delete b; return rc; }
         b->plans.push_back(p);

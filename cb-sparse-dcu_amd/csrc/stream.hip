@@ -26,6 +26,8 @@
 namespace cbs {
 namespace dev {
 
+constexpr int kMultiRhs = 4;   // right-hand sides reduced per chunk load
+
 template <typename A>
 __device__ __forceinline__ A load_partial(const A* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -304,6 +306,163 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t 
     }
 }
 
+// Several right-hand sides at once: each chunk of keys and values is loaded
+// once and reduced for every right-hand side r < nr (at most R), so the slab
+// is read once per tile instead of once per right-hand side.  x is gathered
+// from global memory (L2) for every entry; LDS holds the tile's line table
+// to decode the column of an entry whose key names an LDS slot.  RHS r
+// reads x + r*ldx and writes y + r*ldy; pieces of cut rows go to
+// partial + r*pstride and share one ticket per split.
+template <typename V, typename A, bool COMP, int EPL, int R>
+__device__ __forceinline__ void stream_tile_multi(const StreamArgs<V, A>& a, uint32_t tile, uint32_t* lines,
+                                                  uint64_t ldx, uint64_t ldy, uint64_t pstride, uint32_t nr) {
+    using Acc = typename std::conditional<COMP, f2, A>::type;
+    const uint32_t kZero = a.tile_slots - 1;
+    const int lane = lane_id();
+    const int wave = (int)(threadIdx.x >> 6);
+    {
+        const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1];
+        for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) lines[s] = a.tile_lines[lb + s];
+    }
+    __syncthreads();
+    if ((uint32_t)wave >= a.tile_items) return;
+    const uint32_t item = tile * a.tile_items + wave;
+    if (item >= a.n_items) return;
+    const StreamItem it = a.items[item];
+    const int cb = a.col_bits;
+    const uint32_t cmask = (1u << cb) - 1u;
+    const uint32_t omask = (1u << (31 - cb)) - 1u;
+    const uint32_t split_first = it.first_partial ? a.item_split_first[item] : 0xffffffffu;
+    const uint32_t split_last = it.last_partial ? a.item_split[item] : 0xffffffffu;
+    constexpr int kParts = COMP ? 2 : 1;
+
+    auto emit = [&](bool go, int off, Acc v, bool final_run, uint32_t r) {
+        if (!go) return;
+        uint32_t slot = 0xffffffffu;
+        if (it.first_partial && off == 0) {
+            const StreamSplit sp = a.splits[split_first];
+            slot = sp.partial_base + (item - sp.first_item);
+        } else if (final_run && it.last_partial) {
+            const StreamSplit sp = a.splits[split_last];
+            slot = sp.partial_base + (item - sp.first_item);
+        }
+        if (slot != 0xffffffffu) {
+            A* p = a.partial + (size_t)r * pstride;
+            if constexpr (COMP) {
+                p[2 * slot] = v.hi;
+                p[2 * slot + 1] = v.lo;
+            } else {
+                p[slot] = v;
+            }
+        } else {
+            a.y[(size_t)r * ldy + it.row_base + (uint32_t)off] = finish(v);
+        }
+    };
+    // column of every entry: padding reads column 0 with value 0
+    auto columns = [&](const uint32_t* kk, uint32_t* cc) {
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+            const uint32_t low = kk[j] & cmask;
+            const bool cached = (kk[j] & kKeyCached) != 0;
+            uint32_t c = low;
+            if (cached) c = low == kZero ? 0u : lines[low / kLineElems] * kLineElems + low % kLineElems;
+            cc[j] = c;
+        }
+    };
+    constexpr uint32_t kChunk = kWave * EPL;
+    auto lanes_of = [&](uint32_t e0) { return min(64u, (it.e_end - e0) / EPL); };
+
+    int carry_off = -1;
+    Acc carry[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) carry[r] = Acc{};
+    Chunk<V, EPL> cur, nxt;
+    uint32_t col[EPL];
+    cur.load(a.slab, it.e_begin, lanes_of(it.e_begin), lane);
+    for (uint32_t e0 = it.e_begin; e0 < it.e_end; e0 += kChunk) {
+        const bool more = e0 + kChunk < it.e_end;
+        if (more) nxt.load(a.slab, e0 + kChunk, lanes_of(e0 + kChunk), lane);
+        const int L = (int)min(63u, (it.e_end - e0) / EPL - 1);
+        const bool valid = lane <= L;
+        columns(cur.k, col);
+        int off[EPL];
+        bool ends[EPL];
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) off[j] = valid ? (int)((cur.k[j] >> cb) & omask) : 0x7fffffff;
+#pragma unroll
+        for (int j = 1; j < EPL; ++j) ends[j - 1] = off[j] != off[j - 1];
+        const int o0 = __builtin_amdgcn_readfirstlane(off[0]);
+        const int prev_off = wave_shift<kWaveShr1>(off[EPL - 1], -1);
+        const int next_off = wave_shift<kWaveShl1>(off[0], -1);
+        const bool carried = carry_off >= 0 && o0 == carry_off;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if ((uint32_t)r >= nr) break;
+            const A* x = a.x + (size_t)r * ldx;
+            Acc s[EPL];
+#pragma unroll
+            for (int j = 0; j < EPL; ++j) {
+                const A xv = x[col[j]];
+                if constexpr (COMP) s[j] = two_prod(widen(cur.v[j]), xv);
+                else s[j] = A(widen(cur.v[j])) * xv;
+            }
+            if (carry_off >= 0) {
+                if (carried) {
+                    if (lane == 0) s[0] += carry[r];
+                } else {
+                    emit(lane == 0, carry_off, carry[r], false, r);
+                }
+            }
+#pragma unroll
+            for (int j = 1; j < EPL; ++j)
+                if (!ends[j - 1]) s[j] += s[j - 1];
+            const Acc T = seg_scan_dpp(s[EPL - 1], off[EPL - 1]);
+            const Acc prev_T = wave_shr1(T);
+            bool first = true;
+#pragma unroll
+            for (int j = 0; j < EPL - 1; ++j) {
+                Acc v = s[j];
+                if (first && prev_off == off[0]) v += prev_T;
+                emit(valid && ends[j], off[j], v, false, r);
+                first = first && !ends[j];
+            }
+            emit(valid && lane < L && next_off != off[EPL - 1], off[EPL - 1], T, false, r);
+            carry[r] = read_lane(T, L);
+        }
+        carry_off = __builtin_amdgcn_readlane(off[EPL - 1], L);
+        if (more) cur = nxt;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if ((uint32_t)r < nr) emit(lane == 0, carry_off, carry[r], true, r);
+
+    for (int which = 0; which < 2; ++which) {
+        const uint32_t sid = which == 0 ? split_first : split_last;
+        if (sid == 0xffffffffu || (which == 1 && sid == split_first)) continue;
+        __threadfence();
+        uint32_t ticket = 0;
+        if (lane == 0) ticket = atomicAdd(a.counters + sid, 1u);
+        ticket = __shfl(ticket, 0, kWave);
+        const StreamSplit sp = a.splits[sid];
+        if (ticket == sp.pieces - 1) {
+            __threadfence();
+            if (lane == 0) {
+                for (uint32_t r = 0; r < nr; ++r) {
+                    const A* p = a.partial + (size_t)r * pstride;
+                    Acc sum{};
+                    for (uint32_t q = 0; q < sp.pieces; ++q) {
+                        const uint32_t slot = kParts * (sp.partial_base + q);
+                        if constexpr (COMP) sum += f2{load_partial(p + slot), load_partial(p + slot + 1)};
+                        else sum += load_partial(p + slot);
+                    }
+                    a.y[(size_t)r * ldy + sp.row] = finish(sum);
+                }
+                atomicExch(a.counters + sid, 0u);
+            }
+        }
+    }
+}
+
 template <typename V, typename A, bool COMP, int EPL, bool SDDMM>
 __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -325,7 +484,7 @@ __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A
 // right-hand side of its plan in turn (the tile's slab is then re-read from
 // L2).  Unit 0 of a plan also zeroes the plan's empty rows.
 template <typename V, typename A, bool COMP, int EPL>
-__global__ __launch_bounds__(kMaxStreamThreads) void stream_batch(const BatchPlan<V, A>* plans, uint32_t nplans,
+__global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan<V, A>* plans, uint32_t nplans,
                                                                   uint32_t units) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     A* xs = reinterpret_cast<A*>(lds_raw);
@@ -344,16 +503,31 @@ __global__ __launch_bounds__(kMaxStreamThreads) void stream_batch(const BatchPla
         }
         const BatchPlan<V, A>& bp = plans[lo];
         const uint32_t tile = u - bp.unit_begin;
-        for (uint32_t r = 0; r < bp.nrhs; ++r) {
+        if (tile == 0)
+            for (uint32_t r = 0; r < bp.nrhs; ++r)
+                for (uint32_t t = threadIdx.x; t < bp.args.n_empty_rows; t += blockDim.x)
+                    bp.y[(size_t)r * bp.ldy + bp.args.empty_rows[t]] = A(0);
+        if (tile >= bp.args.n_tiles) continue;
+        if (bp.nrhs == 1) {   // one right-hand side: the LDS x cache
             StreamArgs<V, A> a = bp.args;
-            a.x = bp.x + (size_t)r * bp.ldx;
-            a.y = bp.y + (size_t)r * bp.ldy;
-            a.partial = bp.partial + (size_t)r * bp.partial_stride;
-            a.counters = bp.counters + (size_t)r * bp.counter_stride;
-            if (tile == 0)
-                for (uint32_t t = threadIdx.x; t < a.n_empty_rows; t += blockDim.x) a.y[a.empty_rows[t]] = A(0);
-            if (tile < a.n_tiles) stream_tile<V, A, COMP, EPL, false>(a, tile, xs);
-            __syncthreads();   // LDS is restaged by the next tile
+            a.x = bp.x;
+            a.y = bp.y;
+            a.partial = bp.partial;
+            a.counters = bp.counters;
+            stream_tile<V, A, COMP, EPL, false>(a, tile, xs);
+            __syncthreads();   // LDS is restaged by the next unit
+            continue;
+        }
+        // several: groups of up to kMultiRhs share each chunk load
+        for (uint32_t r0 = 0; r0 < bp.nrhs; r0 += kMultiRhs) {
+            StreamArgs<V, A> a = bp.args;
+            a.x = bp.x + (size_t)r0 * bp.ldx;
+            a.y = bp.y + (size_t)r0 * bp.ldy;
+            a.partial = bp.partial + (size_t)r0 * bp.partial_stride;
+            a.counters = bp.counters + (size_t)r0 * bp.counter_stride;
+            stream_tile_multi<V, A, COMP, EPL, kMultiRhs>(a, tile, reinterpret_cast<uint32_t*>(xs), bp.ldx, bp.ldy,
+                                                          bp.partial_stride, min(bp.nrhs - r0, (uint32_t)kMultiRhs));
+            __syncthreads();
         }
     }
 }

@@ -52,9 +52,9 @@ enum cbspmv_builder { CBSPMV_BUILDER_GATHER = 0, CBSPMV_BUILDER_COMP = 1 };
 enum cbspmv_dtype { CBSPMV_F64 = 0, CBSPMV_F32 = 1, CBSPMV_BF16 = 2 };
 enum cbspmv_accum { CBSPMV_ACC_NATIVE = 0, CBSPMV_ACC_COMPENSATED = 1 };
 enum cbspmv_backward_mode {
-    CBSPMV_BACKWARD_FUSED = 0, /* one pass over A^T writes dx and stored-order dvals,
-                                  then a permutation pass to input order          */
-    CBSPMV_BACKWARD_DIRECT = 1 /* dx over A^T; dvals by a pass over the input COO  */
+    CBSPMV_BACKWARD_DIRECT = 0, /* dx over A^T; dvals by a pass over the input COO  */
+    CBSPMV_BACKWARD_FUSED = 1   /* one pass over A^T writes dx and stored-order dvals,
+                                   then a permutation pass to input order          */
 };
 enum cbspmv_schedule {
     CBSPMV_SCHED_AUTO = 0,    /* pick by plan size / autotuner                       */
