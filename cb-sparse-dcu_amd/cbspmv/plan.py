@@ -213,6 +213,26 @@ class Plan:
         self._check_vec(vals, self.nnz, "vals")
         check(lib().cbspmv_set_values(self._h, ctypes.c_void_p(vals.data_ptr()), _stream_handle(stream)))
 
+    # ---- persistence
+    def save(self, path):
+        """Writes the built plan to a file (cbspmv_save)."""
+        f = _lib.bind("cbspmv_save", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p])
+        check(f(self._h, str(path).encode()))
+
+    @classmethod
+    def load(cls, path):
+        """A plan read back from cbspmv_save's file, without rebuilding."""
+        f = _lib.bind("cbspmv_load", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)])
+        h = ctypes.c_void_p()
+        check(f(str(path).encode(), ctypes.byref(h)))
+        p = cls.__new__(cls)
+        p._h = h
+        p._uploaded = False
+        st = p.stats()
+        p.m, p.n, p.nnz = st["m"], st["n"], st["nnz"]
+        p.dtype = {_lib.F64: "f64", _lib.F32: "f32", _lib.BF16: "bf16"}[int(p.export("plan_dtype")[0])]
+        return p
+
     def bench(self, x):
         """cb_spmv() protocol on host FP64 x: returns (y, runtime_ms, gflops)."""
         x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))

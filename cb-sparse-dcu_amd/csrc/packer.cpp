@@ -592,6 +592,7 @@ long export_array(const Plan& p, const std::string& s, void* dst, size_t cap, in
         return copy_out(v, dst, cap);
     }
     if (s == "stream_col_bits") return copy_out(std::vector<int>{p.col_bits}, dst, cap);
+    if (s == "plan_dtype") return copy_out(std::vector<int>{p.o.dtype}, dst, cap);
     if (s == "stream_value_pos") {   // byte offset in stream_slab of each entry's value
         std::vector<int64_t> v;
         v.reserve(p.s_key.size());

@@ -224,13 +224,28 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t 
             else s[j] = A(widen(cur.v[j])) * xc[j];
         }
         if constexpr (SDDMM) {
-            if (valid) {
-                A dv[EPL];
+            // x of each entry's row: a chunk's rows are few and consecutive,
+            // so lane l loads x[r0 + l] once and each entry takes its row's
+            // value by a lane shuffle; rows 64 or more past r0 load directly
+            const uint32_t r0 = it.row_base + (uint32_t)__builtin_amdgcn_readfirstlane(off[0]);
+            const A xw = a.xrow[min(r0 + (uint32_t)lane, (uint32_t)a.m - 1u)];
+            A dv[EPL];
+            bool far = false;
+#pragma unroll
+            for (int j = 0; j < EPL; ++j) {
+                const uint32_t d = it.row_base + (uint32_t)off[j] - r0;
+                const bool near = valid && d < 64u;
+                far = far || (valid && !near);
+                dv[j] = xc[j] * __shfl(xw, near ? (int)d : 0, kWave);
+            }
+            if (__any(far)) {
 #pragma unroll
                 for (int j = 0; j < EPL; ++j) {
-                    const uint32_t r = min(it.row_base + (uint32_t)off[j], (uint32_t)a.m - 1u);
-                    dv[j] = xc[j] * a.xrow[r];
+                    const uint32_t d = it.row_base + (uint32_t)off[j] - r0;
+                    if (valid && d >= 64u) dv[j] = xc[j] * a.xrow[min(it.row_base + (uint32_t)off[j], (uint32_t)a.m - 1u)];
                 }
+            }
+            if (valid) {
                 u32x4* dst = reinterpret_cast<u32x4*>(a.dvals + e0 + (uint32_t)lane * EPL);
 #pragma unroll
                 for (int q = 0; q < EPL * (int)sizeof(A) / 16; ++q) {

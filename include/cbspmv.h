@@ -133,6 +133,15 @@ int cbspmv_export(const cbspmv_plan* plan, const char* name, void* dst, size_t* 
 /* Byte offset of a slab region in the canonical image ("total" = size). */
 long long cbspmv_layout_offset(const cbspmv_plan* plan, const char* name);
 
+/* ---- persistence (no reference counterpart: the reference rebuilds its
+ * GLOBAL_BLOCK on every run, cb-spmv-dcu/src/main.hip.cpp:46-70) ------------ */
+/* Writes a built plan (builder output, canonical slab image, stream slab) to
+ * one file; cbspmv_load() reads it back without rebuilding.  The format is
+ * versioned and checked on load (magic, version, element sizes, lengths);
+ * it trusts files written by cbspmv_save.  A transposed plan is not saved. */
+int cbspmv_save(const cbspmv_plan* plan, const char* path);
+int cbspmv_load(const char* path, cbspmv_plan** plan);
+
 /* ---- device execution (stream-ordered on the caller's stream) --------------- */
 /* Copies the plan's device slabs to the current device (allocates). */
 int cbspmv_upload(cbspmv_plan* plan, void* stream);
