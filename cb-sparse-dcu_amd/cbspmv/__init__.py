@@ -14,5 +14,7 @@ from .reference_api import (  # noqa: F401
     cb_spmv,
     coo2block_comp,
     coo2block_gather,
+    benchmark_report_dict,
     print_benchmark_report,
+    writeCSV,
 )

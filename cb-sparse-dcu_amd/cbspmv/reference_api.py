@@ -109,3 +109,22 @@ def print_benchmark_report(matrix_name, rows, columns, nonzeros, block, metrics,
     w(f"{sep}\n{'Method':<14}{'Time (ms)':>25}{'GFLOPS':>25}\n{sep}\n")
     w(f"{'CB-SpMV-MI355X':<14}{metrics.runtime:>25.6f}{metrics.gflops:>25.6f}\n{sep}\n")
     w(f"{'Validation':<14}: {'PASS' if result_is_correct else 'FAIL'}\n{div}\n")
+
+
+def writeCSV(filename, metrics, mtx_name, nnz):
+    """utils.h:256-268: appends "matrix,nnz,runtime_ms,gflops" to a CSV file."""
+    with open(filename, "a") as fh:
+        fh.write(f"{mtx_name},{nnz},{metrics.runtime:g},{metrics.gflops:g}\n")
+
+
+def benchmark_report_dict(matrix_name, rows, columns, nonzeros, block, metrics, result_is_correct):
+    """The fields of print_benchmark_report (utils.h:147-201) as a dict, for
+    JSON output."""
+    return {
+        "matrix": matrix_name, "rows": int(rows), "columns": int(columns), "nonzeros": int(nonzeros),
+        "super_sparse_blocks": int(block.super_sparse_blocks), "analyzed_blocks": int(block.analyzed_blocks),
+        "super_sparse_ratio": float(block.super_sparse_ratio), "column_gather": bool(block.gather_flag),
+        "load_balanced": bool(block.load_balanced),
+        "load_stddev": [float(block.load_stddev_before), float(block.load_stddev_after)] if block.load_balanced else None,
+        "runtime_ms": float(metrics.runtime), "gflops": float(metrics.gflops), "validation": bool(result_is_correct),
+    }

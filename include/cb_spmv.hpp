@@ -26,6 +26,7 @@
 #define CB_SPMV_HPP
 
 #include <cstdint>
+#include <cstdio>
 #include <iomanip>
 #include <iostream>
 #include <stdexcept>
@@ -207,6 +208,15 @@ inline void print_benchmark_report(const std::string& matrix_name, int rows, int
       << metrics.gflops << '\n' << separator << '\n' << std::left << std::setw(14) << "Validation" << ": "
       << (result_is_correct ? "\033[32m" : "\033[31m") << (result_is_correct ? "PASS" : "FAIL") << "\033[0m" << '\n'
       << divider << '\n';
+}
+
+// utils.h:256-268: appends "matrix,nnz,runtime_ms,gflops" to a CSV file.
+inline bool writeCSV(const std::string& filename, const PROGRAM_METRICS& metrics, const std::string& mtx_name,
+                     int nnz) {
+    FILE* f = std::fopen(filename.c_str(), "a");
+    if (!f) return false;
+    std::fprintf(f, "%s,%d,%g,%g\n", mtx_name.c_str(), nnz, metrics.runtime, metrics.gflops);
+    return std::fclose(f) == 0;
 }
 
 #endif  // CB_SPMV_HPP
