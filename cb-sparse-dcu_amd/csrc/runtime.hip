@@ -259,6 +259,9 @@ int cbspmv_spmv(cbspmv_plan* plan, const void* d_x, void* d_y, void* stream) {
     if (!plan) return fail(CBSPMV_ERR_ARG, "null plan");
     if (!plan->dev.uploaded) return fail(CBSPMV_ERR_STATE, "plan not uploaded (call cbspmv_upload first)");
     if ((plan->host.m > 0 && !d_y) || (plan->host.nnz > 0 && !d_x)) return fail(CBSPMV_ERR_ARG, "null vector");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != plan->dev.device)
+        return fail(CBSPMV_ERR_STATE, "current device is not the one the plan was uploaded to");
     return cbs::run_spmv(*plan, d_x, d_y, static_cast<hipStream_t>(stream));
 }
 

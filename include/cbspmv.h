@@ -147,7 +147,9 @@ int cbspmv_load(const char* path, cbspmv_plan** plan);
 int cbspmv_upload(cbspmv_plan* plan, void* stream);
 /* y = A x.  d_x has n elements, d_y has m elements, both of the plan's
  * vector type: double for F64 plans, float for F32/BF16 plans.  y is
- * overwritten (no accumulation).  No allocation, no host synchronisation. */
+ * overwritten (no accumulation).  No allocation, no host synchronisation.
+ * The current device must be the one the plan was uploaded to
+ * (CBSPMV_ERR_STATE otherwise). */
 int cbspmv_spmv(cbspmv_plan* plan, const void* d_x, void* d_y, void* stream);
 /* Forces one schedule for the next calls (CBSPMV_SCHED_*); AUTO restores the
  * plan's choice.  Returns CBSPMV_ERR_UNSUPPORTED if the plan lacks it. */

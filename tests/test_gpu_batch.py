@@ -47,8 +47,8 @@ def _check(mats, X, Y, tol):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dtype", ["f64", "f32"])
-def test_batch_parity(gpu, dtype):
+@pytest.mark.parametrize("dtype,comp", [("f64", False), ("f32", False), ("f32", True), ("bf16", True)])
+def test_batch_parity(gpu, dtype, comp):
     import torch
 
     mats = _mats()
@@ -56,7 +56,7 @@ def test_batch_parity(gpu, dtype):
     rng = np.random.default_rng(1)
     plans, X, Y = [], [], []
     for i, (m, n, r, c, v) in enumerate(mats):
-        opts = dict(dtype=dtype, tile_waves=[4, 9, 8][i % 3], item_nnz=[0, 256, 0][i % 3])
+        opts = dict(dtype=dtype, compensated=comp, tile_waves=[4, 9, 8][i % 3], item_nnz=[0, 256, 0][i % 3])
         plans.append(Plan(m, n, r, c, v, **opts))
         k = [1, 3, 8, 16][i % 4]
         X.append(torch.tensor(rng.uniform(-1, 1, (k, n)), dtype=tdt, device=gpu))
@@ -65,7 +65,7 @@ def test_batch_parity(gpu, dtype):
     b = Batch(plans)
     b.spmv(X, Y)
     torch.cuda.synchronize()
-    tol = 1e-12 if dtype == "f64" else 1e-6
+    tol = {"f64": 1e-12, "f32": 1e-6, "bf16": 2.0 ** -8}[dtype]
     _check(mats, X, Y, tol)
     # second call with the same pointers (no descriptor upload), new values
     for x in X:
