@@ -90,6 +90,10 @@ MATRICES = {
     "n65536": lambda: _rand(4000, 65536, 20, seed=11),
     "nnz9999": lambda: (lambda m, n, r, c, v: (m, n, r[:9999], c[:9999], v[:9999]))(*_rand(1500, 3000, 8, seed=12)),
     "nnz10000": lambda: (lambda m, n, r, c, v: (m, n, r[:10000], c[:10000], v[:10000]))(*_rand(1500, 3000, 8, seed=12)),
+    # 22-bit columns leave 9 bits of row offset in a stream key: items span
+    # at most 511 rows
+    "wide_4M": lambda: _rand(2000, 1 << 22, 20, seed=13),
+    "tall_2M": lambda: _rand(1 << 21, 64, 2, seed=14, empty_rows=0.1),
 }
 
 OPTS = {
