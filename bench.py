@@ -380,6 +380,37 @@ def bench_batched(args):
                         "worst_rel_l2_vs_fp64": worst, "autotune_picks": picks}}
 
 
+def bench_mtx(args):
+    """The reference driver's protocol on one .mtx file: its reader (values
+    forced to 1), x from mt19937(42) U[0,1), GLOBAL_BLOCK + coo2block_gather,
+    cb_spmv (validated first launch, 100 warm-up, 1000 timed launches), the
+    report of print_benchmark_report on stderr; validation here is the
+    oracle's FP64 SpMV at 1e-12 relative L2."""
+    import cbspmv
+    from cbspmv import gen
+    from oracle import pyoracle
+
+    m, n, r, c, v = cbspmv.read_mtx(args.mtx)
+    nnz = int(r.size)
+    x = gen.reference_x(n)
+    gb = cbspmv.GLOBAL_BLOCK(m, n, nnz, block_size=args.block_size)
+    cbspmv.coo2block_gather(r, c, v, gb)
+    import numpy as np
+
+    y = np.zeros(m)
+    met = cbspmv.cb_spmv(gb, x, y)
+    err = pyoracle.rel_l2(y, pyoracle.spmv_seq(m, r, c, v, x))
+    name = os.path.basename(args.mtx)
+    cbspmv.print_benchmark_report(name, m, n, nnz, gb, met, err <= 1e-12, file=sys.stderr)
+    b_alg = algorithmic_bytes(m, n, nnz, "f64")
+    ms = met.runtime
+    return {"metric": "SpMV GB/s (CSR-equivalent algorithmic bytes) on one MI355X", "value": round(b_alg / (ms * 1e-3) / 1e9, 2),
+            "unit": "GB/s", "n_gpus": 1, "steps": 1000, "warmup": 100, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": f"Matrix Market file {name}",
+            "config": {"workload": f"mtx_{name}", "m": m, "n": n, "nnz": nnz, "block_size": args.block_size},
+            "gflops": round(met.gflops, 2), "rel_l2_vs_fp64_oracle": err}
+
+
 def read_traffic(workload_key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -404,6 +435,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE comparator")
     ap.add_argument("--workload", default="spmv", choices=["spmv", "precision", "autograd", "batched"])
+    ap.add_argument("--mtx", default=None,
+                    help="run the reference driver's protocol on a Matrix Market file (main.hip.cpp:46-75)")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -426,6 +459,9 @@ def main():
         print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 5}), flush=True)
         return
 
+    if args.mtx:
+        print(json.dumps(bench_mtx(args)), flush=True)
+        return
     if args.workload != "spmv":
         fn = {"precision": bench_precision, "autograd": bench_autograd, "batched": bench_batched}[args.workload]
         print(json.dumps(fn(args)), flush=True)

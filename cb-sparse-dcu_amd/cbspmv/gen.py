@@ -211,3 +211,15 @@ def random_coo(m, n, per_row, seed=0, dup=False, unit=False, empty_rows=0.0):
         cols = np.concatenate([rng.choice(n, size=int(kk), replace=False) for kk in k]) if rows.size else np.zeros(0, np.int64)
     val = np.ones(rows.size) if unit else rng.uniform(-1.0, 1.0, size=rows.size)
     return rows.astype(np.int32), cols.astype(np.int32), val
+
+
+def reference_x(n, seed=42):
+    """x of the reference's benchmark driver (cb-spmv-dcu/src/main.hip.cpp:58-66):
+    std::mt19937(42) with std::uniform_real_distribution<double>(0, 1).  The
+    legacy numpy RandomState seeds MT19937 the same way, and libstdc++'s
+    generate_canonical<double, 53> combines two 32-bit outputs g1, g2 as
+    (g1 + g2 * 2^32) / 2^64 in double arithmetic (clamped below 1)."""
+    g = np.random.RandomState(seed).randint(0, 2 ** 32, size=2 * n, dtype=np.uint64).astype(np.float64)
+    x = (g[0::2] + g[1::2] * 2.0 ** 32) / 2.0 ** 64
+    x[x >= 1.0] = np.nextafter(1.0, 0.0)
+    return x

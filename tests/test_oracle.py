@@ -209,3 +209,23 @@ def test_policy_and_layout_match_oracle(name, mat, row_owned):
         elif reg != "type_per_blk" and reg != "total":
             assert o % 4 == 0
     assert np.array_equal(p.export("image"), ob.array("image"))
+
+
+def test_reference_x_matches_std_mt19937(tmp_path):
+    """gen.reference_x reproduces the reference driver's x
+    (std::mt19937(42), uniform_real_distribution<double>(0, 1),
+    main.hip.cpp:58-66): compared with a C++ program using the standard
+    library directly."""
+    import subprocess
+
+    from cbspmv.gen import reference_x
+
+    src = tmp_path / "x.cpp"
+    src.write_text('#include <random>\n#include <cstdio>\nint main(){std::mt19937 e(42);'
+                   'std::uniform_real_distribution<double> u(0.0, 1.0);'
+                   'for(int i=0;i<1000;i++) std::printf("%.17g\\n", u(e));}\n')
+    exe = tmp_path / "x"
+    subprocess.run(["g++", "-O1", str(src), "-o", str(exe)], check=True)
+    want = np.array([float(t) for t in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                      check=True).stdout.split()])
+    assert np.array_equal(reference_x(1000), want)
