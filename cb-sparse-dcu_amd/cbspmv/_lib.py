@@ -55,7 +55,8 @@ class Opts(ctypes.Structure):
         ("stream_tile_waves", ctypes.c_int),
         ("stream_tile_kib", ctypes.c_int),
         ("backward_mode", ctypes.c_int),
-        ("reserved", ctypes.c_int * 2),
+        ("stream_panels", ctypes.c_int),
+        ("reserved", ctypes.c_int * 1),
     ]
 
 

@@ -29,6 +29,11 @@ _EXPORT_DTYPES = {
     "stream_tiles": np.uint32,
     "stream_tile_lines": np.uint32,
     "stream_empty_rows": np.uint32,
+    "stream_units": np.uint32,
+    "stream_unit_ptr": np.uint32,
+    "stream_tile_rows": np.uint32,
+    "stream_tile_group": np.uint32,
+    "stream_panel_mask": np.uint8,
     "grad_slot": np.uint32,
     "stream_value_pos": np.int64,
     "slot_of": np.uint32,
@@ -47,7 +52,7 @@ _SCHEDULES = {"auto": _lib.SCHED_AUTO, "blocked": _lib.SCHED_BLOCKED, "stream": 
 
 def make_opts(block_size=16, waves=4, row_owned=True, pattern_wave=True, force_coo64=False,
               dense_sixteenths=15, builder="gather", dtype="f64", compensated=False,
-              schedule="auto", item_nnz=0, tile_waves=0, tile_kib=0, backward="direct"):
+              schedule="auto", item_nnz=0, tile_waves=0, tile_kib=0, backward="direct", panels=0):
     o = _lib.Opts()
     lib().cbspmv_default_opts(ctypes.byref(o))
     o.block_size = int(block_size)
@@ -64,6 +69,7 @@ def make_opts(block_size=16, waves=4, row_owned=True, pattern_wave=True, force_c
     o.stream_tile_waves = int(tile_waves)
     o.stream_tile_kib = int(tile_kib)
     o.backward_mode = {"fused": _lib.BACKWARD_FUSED, "direct": _lib.BACKWARD_DIRECT}[backward]
+    o.stream_panels = int(panels)
     return o
 
 

@@ -60,7 +60,7 @@ void fill_args(const cbspmv_plan& p, StreamArgs<V, A>& a) {
     a.tile_lines = static_cast<const uint32_t*>(d.tile_lines.ptr);
     a.empty_rows = static_cast<const uint32_t*>(d.empty.ptr);
     a.n_empty_rows = (uint32_t)h.empty_rows.size();
-    a.n_items = (uint32_t)h.items.size();
+    a.n_items = h.tile_items_count;
     a.n_tiles = (uint32_t)(h.tiles.size() / 2);
     a.tile_items = (uint32_t)h.o.tile_items;
     a.tile_slots = (uint32_t)tile_slots(h.o.tile_bytes, h.o.dtype);
@@ -173,6 +173,14 @@ int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** bat
             delete b;
             return fail(CBSPMV_ERR_UNSUPPORTED, "plans of a batch need stream tile waves <= 10");
         }
+        if (h.panels > 1) {
+            delete b;
+            return fail(CBSPMV_ERR_UNSUPPORTED, "plans of a batch need stream_panels = 1");
+        }
+        if (!cbs::stream_vectors_fit(h)) {
+            delete b;
+            return fail(CBSPMV_ERR_UNSUPPORTED, "the stream schedule needs x and y below 2 GiB each");
+        }
         const int rc = cbs::upload_plan(*p, s);
         if (rc) { delete b; return rc; }
         b->plans.push_back(p);
@@ -188,7 +196,7 @@ int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** bat
     // a persistent grid: as many workgroups as stay resident at once
     int per_cu = cbs::batch_blocks_per_cu(b->dtype, b->comp, b->threads, b->lds);
     if (per_cu <= 0) {   // occupancy query unavailable: bound by LDS and 32 waves per CU
-        const int by_lds = b->lds ? (int)std::max<size_t>(1, (160 * 1024) / b->lds) : 8;
+        const int by_lds = b->lds ? (int)std::max<size_t>(1, (size_t)(160 * 1024) / b->lds) : 8;
         per_cu = std::max(1, std::min(by_lds, 32 / max_items));
     }
     const uint32_t cap = (uint32_t)(cbs::kMI355XCUs * per_cu);

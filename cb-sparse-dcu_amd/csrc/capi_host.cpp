@@ -35,6 +35,9 @@ int build_plan_object(const cbspmv_opts& o, int m, int n, int nnz, const int* ro
         return fail(CBSPMV_ERR_ARG, "unknown value dtype");
     if (o.schedule < CBSPMV_SCHED_AUTO || o.schedule > CBSPMV_SCHED_STREAM)
         return fail(CBSPMV_ERR_ARG, "unknown schedule");
+    if (o.stream_panels != 0 && o.stream_panels != 1 && o.stream_panels != 2 && o.stream_panels != 4 &&
+        o.stream_panels != 8)
+        return fail(CBSPMV_ERR_ARG, "stream_panels must be 0, 1, 2, 4 or 8");
     // stream keys hold column + row-in-strip + 2 flag bits in 32 bits
     if ((long long)n > (1ll << (30 - coord_bits(bs))))
         return fail(CBSPMV_ERR_UNSUPPORTED, "n above 2^(30 - log2 block_size) is not supported");
@@ -56,9 +59,10 @@ int build_plan_object(const cbspmv_opts& o, int m, int n, int nnz, const int* ro
     h.o.item_nnz = o.stream_item_nnz;
     if (o.stream_tile_waves) h.o.tile_items = o.stream_tile_waves;
     if (o.stream_tile_kib) h.o.tile_bytes = o.stream_tile_kib * 1024;
-    if (h.o.tile_items < 1 || h.o.tile_items > kMaxTileItems || h.o.tile_bytes < 4096 || h.o.tile_bytes > 160 * 1024) {
+    h.o.panels = o.stream_panels;
+    if (h.o.tile_items < 1 || h.o.tile_items > kMaxTileItems || h.o.tile_bytes < 4096 || h.o.tile_bytes > kMaxTileBytes) {
         delete p;
-        return fail(CBSPMV_ERR_ARG, "stream tile shape out of range (1..16 waves, 4..160 KiB)");
+        return fail(CBSPMV_ERR_ARG, "stream tile shape out of range (1..16 waves, 4..156 KiB)");
     }
     std::string err;
     int rc;

@@ -33,6 +33,7 @@ struct Options {
     int item_nnz = 0;
     int tile_items = kDefaultTileItemsOpt;
     int tile_bytes = kDefaultTileBytesOpt;
+    int panels = 0;          // column panels of the stream schedule (0 = by x size)
 };
 
 inline int coord_bits(int bs) { int b = 0; while ((1 << b) < bs) ++b; return b; }
@@ -69,6 +70,9 @@ constexpr uint32_t kKeyCached = 0x80000000u;
 // Tile shape: waves (items) per workgroup and LDS bytes of x cache; the
 // defaults give one 8-wave workgroup per CU with 96 KB of its 160 KB of LDS.
 constexpr int kMaxTileItems = 16;
+// LDS x cache per workgroup at most: the 160 KB of a CU less room for the
+// kernels' static LDS
+constexpr int kMaxTileBytes = 156 * 1024;
 constexpr int kStreamCUs = 256;         // MI355X compute units (default tile count)
 // LDS x-cache elements per workgroup; the last slot always holds 0
 inline int tile_slots(int bytes, int dtype) { return bytes / (dtype == CBSPMV_F64 ? 8 : 4); }
@@ -80,6 +84,21 @@ inline int tile_lines(int bytes, int dtype) { return tile_slots(bytes, dtype) / 
 // with G < 64 lanes (an item's last) stores planes of G x 16 bytes.
 inline int stream_lane_entries(int dtype) { return dtype == CBSPMV_F64 ? 4 : 8; }
 inline int stream_val_bytes(int dtype) { return dtype == CBSPMV_F64 ? 8 : dtype == CBSPMV_F32 ? 4 : 2; }
+
+// Column panels.  When x is larger than one XCD's L2 can hold, the entries a
+// tile cannot serve from its LDS x cache (scattered columns) are moved to
+// panel items: panel q holds columns [q w, (q + 1) w), and the workgroups of
+// panel q's units run on the XCDs q, q + P, ... only, so each XCD gathers
+// from a w-column slice of x that stays in its L2.  A unit is the panel-q
+// entries of one tile's rows; it writes per-row partial sums to row r of
+// panel q's output vector.  The panel kernel runs first; each tile then adds
+// the panel sums of its rows to y once its own items are stored.
+constexpr int kXcds = 8;                        // MI355X XCDs (workgroups dealt round-robin)
+constexpr int64_t kPanelL2Bytes = 4ll << 20;    // x up to this size is gathered whole
+constexpr int64_t kPanelBytes = 2ll << 20;      // panel width target (bytes of x)
+constexpr int kPanelWaves = 8;                  // waves (and items) per panel unit
+// stream_panels = 0 picks panels by the size of x only when this is set
+constexpr bool kPanelsByDefault = false;
 
 struct StreamSplit {       // one per row cut into pieces
     uint32_t row;          // matrix row
@@ -147,6 +166,15 @@ struct Plan {
     std::vector<uint32_t> tiles;            // per tile: first line, line count
     std::vector<uint32_t> tile_lines;       // cached x line ids (element / 16)
     uint32_t partial_slots = 0;
+    // column panels (panels > 1): items [0, tile_items_count) belong to the
+    // tiles, the rest to panel units
+    int panels = 1;
+    uint32_t tile_items_count = 0;
+    std::vector<uint32_t> units;       // per unit: item begin, item end, panel, tile
+    std::vector<uint32_t> unit_ptr;    // panel q's units: [unit_ptr[q], unit_ptr[q + 1])
+    std::vector<uint32_t> tile_rows;   // per tile with panel rows: first row, last row
+    std::vector<uint32_t> tile_group;  // per tile: its tile_rows pair or ~0
+    std::vector<uint8_t> panel_mask;   // per row: panels holding entries of the row
 };
 
 // Builder (builder.cpp): bit-exact with coo2block_gather / coo2block_comp.
@@ -155,6 +183,8 @@ int build_plan(Plan& p, int m, int n, int nnz, const int* row, const int* col,
 // Packer (packer.cpp)
 void pack_canonical(Plan& p);
 void build_stream(Plan& p);
+// Column panels the stream schedule uses for this plan (1 = none).
+int stream_panels(const Plan& p);
 long export_array(const Plan& p, const std::string& name, void* dst, size_t cap, int& status);
 // For every input entry k (in_row/in_col order), its index in the padded
 // entry stream (duplicates merged into one stream entry share it).  Returns

@@ -29,6 +29,8 @@ struct DevicePlan {
     // stream slab
     DeviceBuf key, val, items, item_split, item_split_first, splits,
         partial, counters, empty, tiles, tile_lines;
+    // column panels
+    DeviceBuf units, unit_ptr, tile_rows, tile_group, panel_mask, panel_out;
     // value replacement: per stream entry holding input entries, its value's
     // byte offset in the stream slab and its input entries (CSR, input order)
     DeviceBuf sv_pos, sv_ptr, sv_src;
@@ -78,6 +80,7 @@ size_t vec_elem_bytes(const Plan& h);
 void release_device(DevicePlan& d);
 int upload_plan(cbspmv_plan& p, hipStream_t s);
 // Stream schedule launch; with dvals != null also the SDDMM outputs.
+bool stream_vectors_fit(const Plan& h);
 int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s, const void* xrow = nullptr,
                void* dvals = nullptr);
 int run_spmv(cbspmv_plan& p, const void* x, void* y, hipStream_t s);

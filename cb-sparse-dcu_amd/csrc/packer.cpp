@@ -163,18 +163,234 @@ void pack_canonical(Plan& p) {
 // once per SpMV.  Inside a strip, entries are ordered by row, then block
 // order, then in-block order (a stable counting sort), so every row is one
 // contiguous run.
+namespace {
+
+// Stream entries in row order.  zero marks an entry that only makes its row a
+// run (value 0, padding key).
+struct Entries {
+    std::vector<uint32_t> row, col;
+    std::vector<double> val;
+    std::vector<uint8_t> zero;
+    int64_t size() const { return (int64_t)row.size(); }
+    void reserve(size_t n) { row.reserve(n); col.reserve(n); val.reserve(n); zero.reserve(n); }
+    void push(uint32_t r, uint32_t c, double v, uint8_t z) { row.push_back(r); col.push_back(c); val.push_back(v); zero.push_back(z); }
+};
+
+struct Piece { int64_t b, e; uint8_t fp, lp; uint32_t split_first, split_last; };
+
+struct StreamShape {
+    int epl, cb, max_lines;
+    uint32_t omask, zero_slot;
+    size_t tile_items;
+};
+
+StreamShape stream_shape(const Plan& p) {
+    StreamShape s{};
+    s.epl = stream_lane_entries(p.o.dtype);
+    const int slots = tile_slots(p.o.tile_bytes, p.o.dtype);
+    s.max_lines = tile_lines(p.o.tile_bytes, p.o.dtype);
+    s.tile_items = (size_t)p.o.tile_items;
+    s.zero_slot = (uint32_t)slots - 1;
+    s.cb = 1;
+    while ((1ll << s.cb) < (long long)std::max(p.n, slots)) ++s.cb;   // columns and LDS slots
+    s.omask = (1u << (31 - s.cb)) - 1u;                                // row-offset field
+    return s;
+}
+
+// Entries [b0, e0) of E in items of about `target` entries, cut at row
+// boundaries, an item's rows spanning at most omask + 1 row offsets.  With
+// `split`, a row longer than 4 target is cut into pieces of about target
+// entries that meet in the partial buffer (splits; first_item counts from
+// item_base); without, such a row is one item.
+void cut_items(const Entries& E, int64_t b0, int64_t e0, int64_t target, bool split, uint32_t omask,
+               size_t item_base, std::vector<Piece>& pieces, std::vector<StreamSplit>& splits,
+               uint32_t& partial_slots) {
+    const int64_t split_len = 4 * target;
+    int64_t cur = b0, r = b0;
+    while (r < e0) {
+        int64_t re = r + 1;
+        while (re < e0 && E.row[re] == E.row[r]) ++re;
+        const int64_t len = re - r;
+        if (split && len > split_len) {
+            if (cur < r) pieces.push_back({cur, r, 0, 0, ~0u, ~0u});
+            const int64_t np = (len + target - 1) / target;
+            StreamSplit sp{};
+            sp.row = E.row[r];
+            sp.first_item = (uint32_t)(item_base + pieces.size());
+            sp.pieces = (uint32_t)np;
+            sp.partial_base = partial_slots;
+            partial_slots += (uint32_t)np;
+            const uint32_t id = (uint32_t)splits.size();
+            splits.push_back(sp);
+            for (int64_t q = 0; q < np; ++q)
+                pieces.push_back({r + len * q / np, r + len * (q + 1) / np, (uint8_t)(q > 0), (uint8_t)(q + 1 < np),
+                                  q > 0 ? id : ~0u, q + 1 < np ? id : ~0u});
+            cur = re;
+        } else if (r > cur && ((r - cur) + len > target || E.row[re - 1] - E.row[cur] > omask)) {
+            pieces.push_back({cur, r, 0, 0, ~0u, ~0u});
+            cur = r;
+        }
+        r = re;
+    }
+    if (cur < e0) pieces.push_back({cur, e0, 0, 0, ~0u, ~0u});
+}
+
+// Appends the pieces as items, padded to the lane width: fills p.items, the
+// split ids, p.s_val and the per-position column / row offset / padding flag;
+// src[o] is the entry of E at padded position o (-1 for padding).
+void append_items(Plan& p, const Entries& E, const std::vector<Piece>& pieces, int epl, std::vector<uint32_t>& p_col,
+                  std::vector<uint32_t>& p_off, std::vector<uint8_t>& p_zero, std::vector<int64_t>* src) {
+    const size_t I0 = p.items.size(), NI = pieces.size();
+    int64_t at = (int64_t)p_col.size();
+    std::vector<int64_t> out_at(NI + 1, at);
+    for (size_t i = 0; i < NI; ++i) {
+        const int64_t len = pieces[i].e - pieces[i].b;
+        out_at[i + 1] = out_at[i] + (len + epl - 1) / epl * epl;
+    }
+    const size_t padded = (size_t)out_at[NI];
+    p_col.resize(padded);
+    p_off.resize(padded);
+    p_zero.resize(padded, 0);
+    p.s_val.resize(padded, 0.0);
+    if (src) src->resize(padded, -1);
+    p.items.resize(I0 + NI);
+    p.item_split.resize(I0 + NI, ~0u);
+    p.item_split_first.resize(I0 + NI, ~0u);
+    for (size_t i = 0; i < NI; ++i) {
+        const Piece& pc = pieces[i];
+        StreamItem& it = p.items[I0 + i];
+        it = StreamItem{};
+        it.e_begin = (uint32_t)out_at[i];
+        it.e_end = (uint32_t)out_at[i + 1];
+        it.row_base = E.row[pc.b];
+        it.first_partial = pc.fp;
+        it.last_partial = pc.lp;
+        p.item_split_first[I0 + i] = pc.split_first;
+        p.item_split[I0 + i] = pc.split_last;
+        int64_t o = out_at[i];
+        for (int64_t e = pc.b; e < pc.e; ++e, ++o) {
+            p_col[o] = E.col[e];
+            p_off[o] = E.row[e] - it.row_base;
+            p.s_val[o] = E.zero[e] ? 0.0 : E.val[e];
+            p_zero[o] = E.zero[e];
+            if (src) (*src)[o] = E.zero[e] ? -1 : e;
+        }
+        for (; o < out_at[i + 1]; ++o) {   // padding: extends the last run by 0 * 0
+            p_col[o] = 0;
+            p_off[o] = E.row[pc.e - 1] - it.row_base;
+            p_zero[o] = 1;
+        }
+    }
+}
+
+// Tile items of the entries E: items sized so that the tiles (tile_items
+// items each) number at most kStreamCUs by default (every workgroup of a
+// launch runs in one round on a 256-CU MI355X; an explicit item_nnz is used
+// as given), then each tile's hot x lines (16 elements, hit at least twice;
+// the most hit ones when more than fit) and the keys.  Resets the stream
+// tables of p.  src receives the entry of E at each padded position.
+void layout_tiles(Plan& p, const StreamShape& sh, const Entries& E, std::vector<uint32_t>& p_col,
+                  std::vector<uint32_t>& p_off, std::vector<uint8_t>& p_zero, std::vector<int64_t>& src) {
+    const int64_t total = E.size();
+    p.items.clear();
+    p.item_split.clear();
+    p.item_split_first.clear();
+    p.splits.clear();
+    p.partial_slots = 0;
+    p.s_val.clear();
+    p_col.clear(); p_off.clear(); p_zero.clear(); src.clear();
+    std::vector<Piece> pieces;
+    auto cut = [&](int64_t target) {
+        pieces.clear();
+        p.splits.clear();
+        p.partial_slots = 0;
+        cut_items(E, 0, total, target, true, sh.omask, 0, pieces, p.splits, p.partial_slots);
+    };
+    if (p.o.item_nnz > 0) {
+        cut(std::max<int64_t>(64, std::min<int64_t>(p.o.item_nnz, 1 << 22)));
+    } else {
+        const int64_t max_items = (int64_t)kStreamCUs * p.o.tile_items;
+        int64_t target = std::max<int64_t>(1024, total / max_items + 1);
+        for (int round = 0; round < 64; ++round) {
+            cut(target);
+            if ((int64_t)pieces.size() <= max_items || target >= (1 << 22)) break;
+            target = target + target / 16 + 1;
+        }
+    }
+    append_items(p, E, pieces, sh.epl, p_col, p_off, p_zero, &src);
+    const size_t NI = p.items.size();
+    const int64_t padded = (int64_t)p_col.size();
+    p.s_key.assign((size_t)padded, 0);
+    const size_t NT = (NI + sh.tile_items - 1) / sh.tile_items;
+    std::vector<std::vector<uint32_t>> lines(NT);
+#pragma omp parallel
+    {
+        std::vector<uint32_t> ln;
+        std::vector<std::pair<uint32_t, uint32_t>> cnt;   // (count, line)
+#pragma omp for schedule(dynamic, 4)
+        for (long long t = 0; t < (long long)NT; ++t) {
+            const int64_t b = p.items[t * sh.tile_items].e_begin;
+            const int64_t e = p.items[std::min(NI, (size_t)(t + 1) * sh.tile_items) - 1].e_end;
+            ln.clear();
+            for (int64_t q = b; q < e; ++q) if (!p_zero[q]) ln.push_back(p_col[q] / kLineElems);
+            std::sort(ln.begin(), ln.end());
+            cnt.clear();
+            for (size_t q = 0; q < ln.size();) {
+                size_t r = q + 1;
+                while (r < ln.size() && ln[r] == ln[q]) ++r;
+                if (r - q >= 2) cnt.push_back({(uint32_t)(r - q), ln[q]});
+                q = r;
+            }
+            if ((int)cnt.size() > sh.max_lines) {
+                std::nth_element(cnt.begin(), cnt.begin() + sh.max_lines, cnt.end(),
+                                 [](const std::pair<uint32_t, uint32_t>& x, const std::pair<uint32_t, uint32_t>& y) {
+                                     return x.first != y.first ? x.first > y.first : x.second < y.second;
+                                 });
+                cnt.resize(sh.max_lines);
+            }
+            std::vector<uint32_t>& mine = lines[t];
+            mine.clear();
+            for (auto& c : cnt) mine.push_back(c.second);
+            std::sort(mine.begin(), mine.end());
+            for (int64_t q = b; q < e; ++q) {
+                const uint32_t off = (p_off[q] & sh.omask) << sh.cb;
+                if (p_zero[q]) { p.s_key[q] = kKeyCached | off | sh.zero_slot; continue; }
+                const uint32_t col = p_col[q];
+                auto it = std::lower_bound(mine.begin(), mine.end(), col / kLineElems);
+                if (it != mine.end() && *it == col / kLineElems)
+                    p.s_key[q] = kKeyCached | off | ((uint32_t)(it - mine.begin()) * kLineElems + col % kLineElems);
+                else
+                    p.s_key[q] = off | col;
+            }
+        }
+    }
+    p.tiles.assign(2 * NT, 0);
+    p.tile_lines.clear();
+    for (size_t t = 0; t < NT; ++t) {
+        p.tiles[2 * t] = (uint32_t)p.tile_lines.size();
+        p.tiles[2 * t + 1] = (uint32_t)lines[t].size();
+        p.tile_lines.insert(p.tile_lines.end(), lines[t].begin(), lines[t].end());
+    }
+}
+
+}  // namespace
+
+int stream_panels(const Plan& p) {
+    if (p.o.panels > 0) return p.o.panels;
+    if (!kPanelsByDefault || p.o.accum == CBSPMV_ACC_COMPENSATED) return 1;
+    const int64_t x_bytes = (int64_t)p.n * (p.o.dtype == CBSPMV_F64 ? 8 : 4);
+    if (x_bytes <= kPanelL2Bytes) return 1;
+    int P = 2;
+    while (P < kXcds && (int64_t)P * kPanelBytes < x_bytes) P *= 2;
+    return P;
+}
+
 void build_stream(Plan& p) {
     if (p.stream_ready) return;
     const int bs = p.o.bs, bcb = coord_bits(bs);
-    const int epl = stream_lane_entries(p.o.dtype);
-    const int slots = tile_slots(p.o.tile_bytes, p.o.dtype), max_lines = tile_lines(p.o.tile_bytes, p.o.dtype);
-    const size_t tile_items = (size_t)p.o.tile_items;
-    const uint32_t zero_slot = (uint32_t)slots - 1;
-    int cb = 1;
-    while ((1ll << cb) < (long long)std::max(p.n, slots)) ++cb;   // columns and LDS slots
-    p.col_bits = cb;
-    const uint32_t cmask = (1u << cb) - 1u;
-    const uint32_t omask = (1u << (31 - cb)) - 1u;   // row-offset field
+    const StreamShape sh = stream_shape(p);
+    const int epl = sh.epl;
+    p.col_bits = sh.cb;
 
     // blocks of each strip in final block order (pad blocks skipped)
     std::vector<int> sb_ptr(p.gb_m + 1, 0), sb_list;
@@ -204,8 +420,11 @@ void build_stream(Plan& p) {
         strip_ptr[b + 1] = strip_ptr[b] + c;
     }
     const int64_t total = strip_ptr[p.gb_m];
-    std::vector<uint32_t> e_row(total), e_col(total);
-    std::vector<double> e_val(total);
+    Entries E;
+    E.row.resize(total);
+    E.col.resize(total);
+    E.val.resize(total);
+    E.zero.assign(total, 0);
 
     const uint32_t rmask = (1u << bcb) - 1;
 #pragma omp parallel
@@ -255,9 +474,9 @@ void build_stream(Plan& p) {
             const int64_t base = strip_ptr[b];
             for (size_t k = 0; k < tr.size(); ++k) {
                 const int64_t at = base + row_cnt[tr[k]]++;
-                e_row[at] = (uint32_t)(b * bs + tr[k]);
-                e_col[at] = tk[k];
-                e_val[at] = tv[k];
+                E.row[at] = (uint32_t)(b * bs + tr[k]);
+                E.col[at] = tk[k];
+                E.val[at] = tv[k];
             }
         }
     }
@@ -266,158 +485,137 @@ void build_stream(Plan& p) {
     p.empty_rows.clear();
     {
         std::vector<uint8_t> has(p.m, 0);
-        for (int64_t e = 0; e < total; ++e) if (e_row[e] < (uint32_t)p.m) has[e_row[e]] = 1;
+        for (int64_t e = 0; e < total; ++e) if (E.row[e] < (uint32_t)p.m) has[E.row[e]] = 1;
         for (int r = 0; r < p.m; ++r) if (!has[r]) p.empty_rows.push_back((uint32_t)r);
     }
 
-    // ---- items: row runs packed into items of about `target` entries.  By
-    // default the target is chosen so that the tiles (tile_items items each)
-    // number at most kStreamCUs, i.e. every workgroup of a launch runs in one
-    // round on a 256-CU MI355X; an explicit item_nnz is used as given.
-    struct Piece { int64_t b, e; uint8_t fp, lp; uint32_t split_first, split_last; };
-    std::vector<Piece> pieces;
-    auto cut = [&](int64_t target) {
-        const int64_t split_len = 4 * target;
-        pieces.clear();
-        p.splits.clear();
-        p.partial_slots = 0;
-        int64_t cur = 0;
-        int64_t r = 0;
-        while (r < total) {
-            int64_t re = r + 1;
-            while (re < total && e_row[re] == e_row[r]) ++re;
-            const int64_t len = re - r;
-            if (len > split_len) {
-                if (cur < r) pieces.push_back({cur, r, 0, 0, ~0u, ~0u});
-                const int64_t np = (len + target - 1) / target;
-                StreamSplit sp{};
-                sp.row = e_row[r];
-                sp.first_item = (uint32_t)pieces.size();
-                sp.pieces = (uint32_t)np;
-                sp.partial_base = p.partial_slots;
-                p.partial_slots += (uint32_t)np;
-                const uint32_t id = (uint32_t)p.splits.size();
-                p.splits.push_back(sp);
-                for (int64_t q = 0; q < np; ++q)
-                    pieces.push_back({r + len * q / np, r + len * (q + 1) / np, (uint8_t)(q > 0), (uint8_t)(q + 1 < np),
-                                      q > 0 ? id : ~0u, q + 1 < np ? id : ~0u});
-                cur = re;
-            } else if (r > cur && ((r - cur) + len > target || e_row[re - 1] - e_row[cur] > omask)) {
-                pieces.push_back({cur, r, 0, 0, ~0u, ~0u});
-                cur = r;
-            }
-            r = re;
-        }
-        if (cur < total) pieces.push_back({cur, total, 0, 0, ~0u, ~0u});
-    };
-    if (p.o.item_nnz > 0) {
-        cut(std::max<int64_t>(64, std::min<int64_t>(p.o.item_nnz, 1 << 22)));
-    } else {
-        const int64_t max_items = (int64_t)kStreamCUs * p.o.tile_items;
-        int64_t target = std::max<int64_t>(1024, total / max_items + 1);
-        for (int round = 0; round < 64; ++round) {
-            cut(target);
-            if ((int64_t)pieces.size() <= max_items || target >= (1 << 22)) break;
-            target = target + target / 16 + 1;
-        }
-    }
-    // ---- padded entry stream and item table
-    const size_t NI = pieces.size();
-    p.items.assign(NI, StreamItem{});
-    p.item_split.assign(NI, ~0u);
-    p.item_split_first.assign(NI, ~0u);
-    std::vector<int64_t> out_at(NI + 1, 0);
-    for (size_t i = 0; i < NI; ++i) {
-        const int64_t len = pieces[i].e - pieces[i].b;
-        out_at[i + 1] = out_at[i] + (len + epl - 1) / epl * epl;
-    }
-    const int64_t padded = out_at[NI];
-    std::vector<uint32_t> p_col(padded), p_off(padded);
-    std::vector<uint8_t> p_zero(padded, 0);
-    p.s_val.assign(padded, 0.0);
-    for (size_t i = 0; i < NI; ++i) {
-        const Piece& pc = pieces[i];
-        StreamItem& it = p.items[i];
-        it.e_begin = (uint32_t)out_at[i];
-        it.e_end = (uint32_t)out_at[i + 1];
-        it.row_base = e_row[pc.b];
-        it.first_partial = pc.fp;
-        it.last_partial = pc.lp;
-        p.item_split_first[i] = pc.split_first;
-        p.item_split[i] = pc.split_last;
-        int64_t o = out_at[i];
-        for (int64_t e = pc.b; e < pc.e; ++e, ++o) {
-            p_col[o] = e_col[e];
-            p_off[o] = e_row[e] - it.row_base;
-            p.s_val[o] = e_val[e];
-        }
-        for (; o < out_at[i + 1]; ++o) {   // padding: extends the last run by 0 * 0
-            p_col[o] = 0;
-            p_off[o] = e_row[pc.e - 1] - it.row_base;
-            p_zero[o] = 1;
-        }
-    }
-    std::vector<uint32_t>().swap(e_row);
-    std::vector<uint32_t>().swap(e_col);
-    std::vector<double>().swap(e_val);
+    std::vector<uint32_t> p_col, p_off;
+    std::vector<uint8_t> p_zero;
+    std::vector<int64_t> src;
+    layout_tiles(p, sh, E, p_col, p_off, p_zero, src);
 
-    // ---- tiles: tile_items consecutive items; hot x lines staged in LDS
-    p.s_key.assign(padded, 0);
-    const size_t NT = (NI + tile_items - 1) / tile_items;
-    std::vector<std::vector<uint32_t>> lines(NT);
-#pragma omp parallel
-    {
-        std::vector<uint32_t> ln;
-        std::vector<std::pair<uint32_t, uint32_t>> cnt;   // (count, line)
-#pragma omp for schedule(dynamic, 4)
-        for (long long t = 0; t < (long long)NT; ++t) {
-            const int64_t b = p.items[t * tile_items].e_begin;
-            const int64_t e = p.items[std::min(NI, (size_t)(t + 1) * tile_items) - 1].e_end;
-            ln.clear();
-            for (int64_t q = b; q < e; ++q) if (!p_zero[q]) ln.push_back(p_col[q] / kLineElems);
-            std::sort(ln.begin(), ln.end());
-            cnt.clear();
-            for (size_t q = 0; q < ln.size();) {
-                size_t r = q + 1;
-                while (r < ln.size() && ln[r] == ln[q]) ++r;
-                if (r - q >= 2) cnt.push_back({(uint32_t)(r - q), ln[q]});
-                q = r;
+    // ---- column panels: the entries whose x line is not staged in LDS are
+    // moved to panel items, grouped by (tile, panel), so that the workgroups
+    // of one XCD gather x from one panel of columns only (held in that XCD's
+    // L2).  Rows cut into pieces keep all their entries in the tiles.  A row
+    // left without tile entries keeps a zero entry there, so its tile stores
+    // its y before adding the panel sums.
+    p.panels = 1;
+    p.tile_items_count = (uint32_t)p.items.size();
+    p.units.clear(); p.unit_ptr.clear(); p.tile_rows.clear(); p.tile_group.clear(); p.panel_mask.clear();
+    const int P = stream_panels(p);
+    if (P > 1) {
+        const int64_t padded = (int64_t)p_col.size();
+        std::vector<uint8_t> uncached(total, 0), split_row(p.m, 0);
+        for (const StreamSplit& sp : p.splits) split_row[sp.row] = 1;
+        int64_t n_out = 0;
+        for (int64_t o = 0; o < padded; ++o) {
+            if (src[o] < 0 || (p.s_key[o] & kKeyCached)) continue;
+            if (split_row[E.row[src[o]]]) continue;
+            uncached[src[o]] = 1;
+            ++n_out;
+        }
+        if (n_out > 0) {
+            const uint32_t width = (uint32_t)(((int64_t)p.n + P - 1) / P + kLineElems - 1) / kLineElems * kLineElems;
+            Entries T, U;   // tile entries, panel entries
+            T.reserve((size_t)(total - n_out + 64));
+            U.reserve((size_t)n_out);
+            for (int64_t r0 = 0; r0 < total;) {
+                int64_t r1 = r0 + 1;
+                while (r1 < total && E.row[r1] == E.row[r0]) ++r1;
+                bool kept = false;
+                for (int64_t q = r0; q < r1; ++q) {
+                    if (uncached[q]) U.push(E.row[q], E.col[q], E.val[q], 0);
+                    else { T.push(E.row[q], E.col[q], E.val[q], E.zero[q]); kept = true; }
+                }
+                if (!kept) T.push(E.row[r0], 0, 0.0, 1);
+                r0 = r1;
             }
-            if ((int)cnt.size() > max_lines) {
-                std::nth_element(cnt.begin(), cnt.begin() + max_lines, cnt.end(),
-                                 [](const std::pair<uint32_t, uint32_t>& x, const std::pair<uint32_t, uint32_t>& y) {
-                                     return x.first != y.first ? x.first > y.first : x.second < y.second;
-                                 });
-                cnt.resize(max_lines);
+            E = Entries();
+            layout_tiles(p, sh, T, p_col, p_off, p_zero, src);
+            p.panels = P;
+            p.tile_items_count = (uint32_t)p.items.size();
+            // tile of every row (rows of U always have tile entries)
+            const size_t NT = p.tiles.size() / 2;
+            std::vector<uint32_t> row_tile(p.m, ~0u);
+            for (size_t i = 0; i < p.items.size(); ++i) {
+                const StreamItem& it = p.items[i];
+                for (uint32_t e = it.e_begin; e < it.e_end; ++e) {
+                    const uint32_t r = it.row_base + p_off[e];
+                    if (row_tile[r] == ~0u) row_tile[r] = (uint32_t)(i / sh.tile_items);
+                }
             }
-            std::vector<uint32_t>& mine = lines[t];
-            mine.clear();
-            for (auto& c : cnt) mine.push_back(c.second);
-            std::sort(mine.begin(), mine.end());
-            for (int64_t q = b; q < e; ++q) {
-                const uint32_t off = (p_off[q] & omask) << cb;
-                if (p_zero[q]) { p.s_key[q] = kKeyCached | off | zero_slot; continue; }
-                const uint32_t col = p_col[q];
-                auto it = std::lower_bound(mine.begin(), mine.end(), col / kLineElems);
-                if (it != mine.end() && *it == col / kLineElems)
-                    p.s_key[q] = kKeyCached | off | ((uint32_t)(it - mine.begin()) * kLineElems + col % kLineElems);
-                else
-                    p.s_key[q] = off | col;
+            // U in (tile, panel) order, rows in order inside (a stable counting sort)
+            const int64_t nu = U.size();
+            std::vector<int64_t> key_ptr(NT * P + 1, 0);
+            std::vector<uint32_t> ukey(nu);
+            for (int64_t q = 0; q < nu; ++q) {
+                ukey[q] = row_tile[U.row[q]] * (uint32_t)P + U.col[q] / width;
+                key_ptr[ukey[q] + 1]++;
+            }
+            for (size_t k = 0; k < NT * P; ++k) key_ptr[k + 1] += key_ptr[k];
+            Entries S;
+            S.row.resize(nu); S.col.resize(nu); S.val.resize(nu); S.zero.assign(nu, 0);
+            {
+                std::vector<int64_t> at(key_ptr.begin(), key_ptr.end() - 1);
+                for (int64_t q = 0; q < nu; ++q) {
+                    const int64_t d = at[ukey[q]]++;
+                    S.row[d] = U.row[q]; S.col[d] = U.col[q]; S.val[d] = U.val[q];
+                }
+            }
+            // units, dealt by panel: unit_ptr[q] .. unit_ptr[q + 1] are panel q's
+            p.panel_mask.assign(p.m, 0);
+            for (int64_t q = 0; q < nu; ++q) p.panel_mask[S.row[q]] |= (uint8_t)(1u << (S.col[q] / width));
+            p.tile_group.assign(NT, ~0u);
+            for (size_t t = 0; t < NT; ++t) {
+                if (key_ptr[t * P + P] == key_ptr[t * P]) continue;
+                p.tile_group[t] = (uint32_t)(p.tile_rows.size() / 2);
+                const StreamItem& first = p.items[t * sh.tile_items];
+                const size_t last_i = std::min(p.items.size(), (t + 1) * sh.tile_items) - 1;
+                const StreamItem& last = p.items[last_i];
+                p.tile_rows.push_back(first.row_base);
+                p.tile_rows.push_back(last.row_base + p_off[last.e_end - 1]);
+            }
+            p.unit_ptr.assign(P + 1, 0);
+            std::vector<Piece> pieces;
+            std::vector<StreamSplit> no_splits;
+            uint32_t no_slots = 0;
+            for (int q = 0; q < P; ++q) {
+                for (size_t t = 0; t < NT; ++t) {
+                    const int64_t b = key_ptr[t * P + q], e = key_ptr[t * P + q + 1];
+                    if (b == e) continue;
+                    // at most kPanelWaves items: one per wave of the unit
+                    int64_t target = std::max<int64_t>(256, (e - b + kPanelWaves - 1) / kPanelWaves);
+                    for (;;) {
+                        pieces.clear();
+                        cut_items(S, b, e, target, false, sh.omask, 0, pieces, no_splits, no_slots);
+                        if (pieces.size() <= (size_t)kPanelWaves || target >= e - b) break;
+                        target += target / 8 + 1;
+                    }
+                    const uint32_t ib = (uint32_t)p.items.size();
+                    append_items(p, S, pieces, epl, p_col, p_off, p_zero, nullptr);
+                    p.units.push_back(ib);
+                    p.units.push_back((uint32_t)p.items.size());
+                    p.units.push_back((uint32_t)q);
+                    p.units.push_back((uint32_t)t);
+                }
+                p.unit_ptr[q + 1] = (uint32_t)(p.units.size() / 4);
+            }
+            // panel keys: column (never staged), padding as in the tiles
+            const size_t start = p.s_key.size();
+            p.s_key.resize(p_col.size());
+            for (size_t o = start; o < p_col.size(); ++o) {
+                const uint32_t off = (p_off[o] & sh.omask) << sh.cb;
+                p.s_key[o] = p_zero[o] ? (kKeyCached | off | sh.zero_slot) : (off | p_col[o]);
             }
         }
     }
-    p.tiles.assign(2 * NT, 0);
-    p.tile_lines.clear();
-    for (size_t t = 0; t < NT; ++t) {
-        p.tiles[2 * t] = (uint32_t)p.tile_lines.size();
-        p.tiles[2 * t + 1] = (uint32_t)lines[t].size();
-        p.tile_lines.insert(p.tile_lines.end(), lines[t].begin(), lines[t].end());
-    }
-    (void)cmask;
 
     // ---- device slab: per chunk, key planes then value planes (16 B/lane)
+    const size_t NI = p.items.size();
+    const int64_t padded = (int64_t)p_col.size();
     const int vb = stream_val_bytes(p.o.dtype);
-    const int nk = epl * 4 / 16, nv = epl * vb / 16;
+    const int nk = epl * 4 / 16;
     const int64_t eb = 4 + vb;
     p.s_slab.assign((size_t)(padded * eb), 0);
 #pragma omp parallel for schedule(dynamic, 64)
@@ -448,7 +646,6 @@ void build_stream(Plan& p) {
                 }
         }
     }
-    (void)nv;
     p.stream_ready = true;
 }
 
@@ -480,7 +677,8 @@ bool stream_entry_slots(const Plan& p, std::vector<uint32_t>& slot) {
     const uint32_t cmask = (1u << cb) - 1u, omask = (1u << (31 - cb)) - 1u;
     const uint32_t zero_slot = (uint32_t)tile_slots(p.o.tile_bytes, p.o.dtype) - 1u;
     const size_t NI = p.items.size();
-    // real entries in stream order; rows are non-decreasing along the stream
+    // real entries in stream order (rows are non-decreasing inside the tile
+    // items and inside the panel items), then grouped by row
     std::vector<uint32_t> srow, scol, spos;
     srow.reserve((size_t)nnz);
     scol.reserve((size_t)nnz);
@@ -507,7 +705,17 @@ bool stream_entry_slots(const Plan& p, std::vector<uint32_t>& slot) {
     }
     for (int r : p.in_row) iptr[r + 1]++;
     for (int r = 0; r < m; ++r) { sptr[r + 1] += sptr[r]; iptr[r + 1] += iptr[r]; }
-    for (size_t q = 1; q < srow.size(); ++q) if (srow[q] < srow[q - 1]) return false;
+    {   // stable counting sort by row
+        std::vector<uint32_t> c2(srow.size()), p2(srow.size());
+        std::vector<int64_t> at(sptr.begin(), sptr.end() - 1);
+        for (size_t q = 0; q < srow.size(); ++q) {
+            const int64_t d = at[srow[q]]++;
+            c2[d] = scol[q];
+            p2[d] = spos[q];
+        }
+        scol.swap(c2);
+        spos.swap(p2);
+    }
     bool ok = true;
 #pragma omp parallel
     {
@@ -592,6 +800,12 @@ long export_array(const Plan& p, const std::string& s, void* dst, size_t cap, in
         return copy_out(v, dst, cap);
     }
     if (s == "stream_col_bits") return copy_out(std::vector<int>{p.col_bits}, dst, cap);
+    if (s == "stream_panels") return copy_out(std::vector<int>{p.panels, (int)p.tile_items_count}, dst, cap);
+    if (s == "stream_units") return copy_out(p.units, dst, cap);
+    if (s == "stream_unit_ptr") return copy_out(p.unit_ptr, dst, cap);
+    if (s == "stream_tile_rows") return copy_out(p.tile_rows, dst, cap);
+    if (s == "stream_tile_group") return copy_out(p.tile_group, dst, cap);
+    if (s == "stream_panel_mask") return copy_out(p.panel_mask, dst, cap);
     if (s == "plan_dtype") return copy_out(std::vector<int>{p.o.dtype}, dst, cap);
     if (s == "stream_value_pos") {   // byte offset in stream_slab of each entry's value
         std::vector<int64_t> v;

@@ -23,7 +23,7 @@ namespace cbs {
 namespace {
 
 constexpr char kMagic[16] = "CBSPMV-PLAN";
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;
 constexpr uint32_t kBom = 0x01020304u;
 
 struct Writer {
@@ -84,6 +84,8 @@ void members(IO& io, Plan& p, cbspmv_opts& o) {
     io.vec(p.s_key); io.vec(p.s_val); io.vec(p.s_slab); io.vec(p.items);
     io.vec(p.item_split); io.vec(p.item_split_first); io.vec(p.splits); io.vec(p.empty_rows);
     io.vec(p.tiles); io.vec(p.tile_lines); io.pod(p.partial_slots);
+    io.pod(p.panels); io.pod(p.tile_items_count);
+    io.vec(p.units); io.vec(p.unit_ptr); io.vec(p.tile_rows); io.vec(p.tile_group); io.vec(p.panel_mask);
 }
 
 void write_offsets(Writer& w, const Plan& p) {

@@ -101,6 +101,63 @@ struct Chunk {
     }
 };
 
+// ---- buffer instructions (raw buffer resources, gfx9 layout).  A lane whose
+// offset is at or past the resource's byte size reads zeros and writes
+// nothing, without a memory request, so a lane that has nothing to load or
+// store passes kOutOfRange instead of branching around the instruction.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;       // dword 3 of a raw buffer resource
+constexpr uint32_t kOutOfRange = 0x80000000u; // every resource here is < 2 GiB
+constexpr int kNonTemporal = 2;               // cache policy bit: nt
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRsrcWord3);
+}
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+template <typename A>
+__device__ __forceinline__ A buffer_load(rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ double buffer_load<double>(rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ float buffer_load<float>(rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <typename A>
+__device__ __forceinline__ void buffer_store(A v, rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ void buffer_store<double>(double v, rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+}
+template <>
+__device__ __forceinline__ void buffer_store<float>(float v, rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
+}
+
+// One chunk of an item through its buffer resource: r0 is the chunk's first
+// entry relative to the item (ne entries); a chunk past the end reads zeros.
+template <typename V, int EPL>
+struct LaneChunk {
+    static constexpr int NK = EPL * 4 / 16;
+    static constexpr int NV = EPL * (int)sizeof(V) / 16;
+    uint32_t k[EPL];
+    V v[EPL];
+    __device__ __forceinline__ void load(rsrc_t rs, uint32_t r0, uint32_t ne, int lane) {
+        const uint32_t G = r0 < ne ? min(64u, (ne - r0) / EPL) : 64u;
+        const uint32_t base = r0 < ne ? r0 * (4 + (uint32_t)sizeof(V)) + min((uint32_t)lane, G - 1) * 16 : kOutOfRange;
+#pragma unroll
+        for (int p = 0; p < NK; ++p) {
+            const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(rs, base + p * G * 16, 0, kNonTemporal);
+            k[4 * p] = w.x; k[4 * p + 1] = w.y; k[4 * p + 2] = w.z; k[4 * p + 3] = w.w;
+        }
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(rs, base + (NK + q) * G * 16, 0, kNonTemporal);
+            __builtin_memcpy(&v[q * (16 / sizeof(V))], &w, 16);
+        }
+    }
+};
+
 // DPP whole-wave shifts (GFX9): wave_shl:1 (lane i reads i+1), wave_shr:1
 // (lane i reads i-1); the lane without a source reads `old`.
 template <int CTRL>
@@ -112,52 +169,47 @@ __device__ __forceinline__ double wave_shr1(double v) { return dpp<kWaveShr1, 0x
 __device__ __forceinline__ float wave_shr1(float v) { return dpp<kWaveShr1, 0xf>(v); }
 __device__ __forceinline__ f2 wave_shr1(f2 v) { return f2{dpp<kWaveShr1, 0xf>(v.hi), dpp<kWaveShr1, 0xf>(v.lo)}; }
 
-// One tile of the stream: stage its hot x lines in LDS, then each of its
-// waves streams one item.  Every thread of the workgroup calls this (it
-// holds a workgroup barrier); waves without an item only stage.
-template <typename V, typename A, bool COMP, int EPL, bool SDDMM>
-__device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t tile, A* xs) {
-    using Acc = typename std::conditional<COMP, f2, A>::type;
-    const uint32_t SLOTS = a.tile_slots;
-    const uint32_t kZero = SLOTS - 1;
+// Stages a tile's hot x lines in LDS, PER slots per thread per pass with all
+// line-id loads, then all x loads, issued back to back (16 consecutive
+// threads read one line); the last slot stays 0 for the padding entries.
+// Every thread of the workgroup calls this; it ends with a workgroup barrier.
+template <typename V, typename A>
+__device__ __forceinline__ void stage_tile(const StreamArgs<V, A>& a, uint32_t tile, A* xs) {
+    constexpr int PER = 8;
     const uint32_t nthreads = blockDim.x;
-    const int lane = lane_id();
-    const int wave = (int)(threadIdx.x >> 6);
-
-    // ---- stage this tile's hot x lines in LDS, PER slots per thread per
-    // pass with all line-id loads, then all x loads, issued back to back (16
-    // consecutive threads read one line); the last slot stays 0 for the
-    // padding entries
-    {
-        constexpr int PER = 8;
-        const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1] * kLineElems;
-        for (uint32_t s0 = threadIdx.x; s0 < cnt; s0 += PER * nthreads) {
-            uint32_t lid[PER];
+    const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1] * kLineElems;
+    for (uint32_t s0 = threadIdx.x; s0 < cnt; s0 += PER * nthreads) {
+        uint32_t lid[PER];
 #pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                const uint32_t s = s0 + j * nthreads;
-                lid[j] = s < cnt ? a.tile_lines[lb + s / kLineElems] : 0u;
-            }
-            A xv[PER];
-#pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                const uint32_t s = s0 + j * nthreads;
-                const uint32_t c = lid[j] * kLineElems + (s % kLineElems);
-                xv[j] = (s < cnt && c < (uint32_t)a.n) ? a.x[c] : A(0);
-            }
-#pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                const uint32_t s = s0 + j * nthreads;
-                if (s < cnt) xs[s] = xv[j];
-            }
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t s = s0 + j * nthreads;
+            lid[j] = s < cnt ? a.tile_lines[lb + s / kLineElems] : 0u;
         }
-        if (threadIdx.x == 0) xs[kZero] = A(0);
+        A xv[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t s = s0 + j * nthreads;
+            const uint32_t c = lid[j] * kLineElems + (s % kLineElems);
+            xv[j] = (s < cnt && c < (uint32_t)a.n) ? a.x[c] : A(0);
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t s = s0 + j * nthreads;
+            if (s < cnt) xs[s] = xv[j];
+        }
     }
+    if (threadIdx.x == 0) xs[a.tile_slots - 1] = A(0);
     __syncthreads();
+}
 
-    if ((uint32_t)wave >= a.tile_items) return;
-    const uint32_t item = tile * a.tile_items + wave;
-    if (item >= a.n_items) return;
+// One item, streamed by one wave: y rows go to yout (y, or a panel's partial
+// sums).  With LDS, keys flagged as staged read x from the tile's LDS cache
+// xs; without (panel items), such keys are padding and read 0.
+template <typename V, typename A, bool COMP, int EPL, bool SDDMM, bool LDS>
+__device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t item, const A* xs, A* yout) {
+    using Acc = typename std::conditional<COMP, f2, A>::type;
+    const uint32_t kZero = a.tile_slots - 1;
+    const int lane = lane_id();
     const StreamItem it = a.items[item];
     const int cb = a.col_bits;
     const uint32_t cmask = (1u << cb) - 1u;
@@ -167,8 +219,8 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t 
 
     // A finished run (row offset `off`) goes to y, or to the partial buffer
     // when it is a piece of a cut row (the item's first run, offset 0, or its
-    // last run).
-    // Pieces keep their full accumulator (hi and lo of a compensated sum).
+    // last run).  Pieces keep their full accumulator (hi and lo of a
+    // compensated sum).
     constexpr int kParts = COMP ? 2 : 1;
     auto put_partial = [&](uint32_t slot, Acc v) {
         if constexpr (COMP) {
@@ -178,62 +230,60 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t 
             a.partial[slot] = v;
         }
     };
-    auto emit = [&](bool go, int off, Acc v, bool final_run) {
-        if (!go) return;
-        if (it.first_partial && off == 0) {
-            const StreamSplit sp = a.splits[split_first];
-            put_partial(sp.partial_base + (item - sp.first_item), v);
-        } else if (final_run && it.last_partial) {
-            const StreamSplit sp = a.splits[split_last];
-            put_partial(sp.partial_base + (item - sp.first_item), v);
-        } else {
-            a.y[it.row_base + (uint32_t)off] = finish(v);
-        }
-    };
-    auto gather = [&](const uint32_t* kk, A* xx) {
+
+    // Every memory instruction of the loop below is issued by all lanes and on
+    // every pass (buffer instructions whose out-of-range lanes neither read nor
+    // write), so the compiler can count them and wait for exactly the load a
+    // value comes from: chunk c + RING - 1 is loaded and the x of chunk c + 1
+    // gathered while chunk c is reduced.
+    constexpr uint32_t EB = 4 + sizeof(V);
+    constexpr uint32_t kChunk = kWave * EPL;
+    constexpr int RING = 3;
+    const uint32_t ne = it.e_end - it.e_begin;
+    const rsrc_t rs = make_rsrc(a.slab + (size_t)it.e_begin * EB, ne * EB);
+    const rsrc_t rx = make_rsrc(a.x, (uint32_t)a.n * (uint32_t)sizeof(A));
+    const uint32_t item_rows = min((uint32_t)a.m - it.row_base, omask + 1u);
+    const rsrc_t ry = make_rsrc(yout + it.row_base, item_rows * (uint32_t)sizeof(A));
+    const bool first_piece = it.first_partial != 0;
+
+    auto gather = [&](const LaneChunk<V, EPL>& c, A* g, bool live) {
 #pragma unroll
         for (int j = 0; j < EPL; ++j) {
-            const uint32_t c = kk[j] & cmask;
-            const bool cached = (kk[j] & kKeyCached) != 0;
-            A g = A(0);
-            if (!cached) g = a.x[c];
-            const A l = xs[min(c, kZero)];
-            xx[j] = cached ? l : g;
+            const bool cached = (c.k[j] & kKeyCached) != 0;
+            g[j] = buffer_load<A>(rx, live && !cached ? (c.k[j] & cmask) * (uint32_t)sizeof(A) : kOutOfRange);
         }
     };
-    constexpr uint32_t kChunk = kWave * EPL;
-    auto lanes_of = [&](uint32_t e0) { return min(64u, (it.e_end - e0) / EPL); };
 
     int carry_off = -1;
     Acc carry{};
-    Chunk<V, EPL> cur, nxt;
-    A xc[EPL];
-    cur.load(a.slab, it.e_begin, lanes_of(it.e_begin), lane);
-    gather(cur.k, xc);
-    for (uint32_t e0 = it.e_begin; e0 < it.e_end; e0 += kChunk) {
-        const bool more = e0 + kChunk < it.e_end;
-        if (more) nxt.load(a.slab, e0 + kChunk, lanes_of(e0 + kChunk), lane);   // in flight while this chunk is reduced
-        const int L = (int)min(63u, (it.e_end - e0) / EPL - 1);   // last lane with entries
+    Acc held{};            // the item's first run when it is a piece of a cut row
+    bool has_held = false;
+    auto reduce = [&](const LaneChunk<V, EPL>& c, const A* g, uint32_t r0) {
+        const int L = (int)min(63u, (ne - r0) / EPL - 1);   // last lane with entries
         const bool valid = lane <= L;
         int off[EPL];
         Acc s[EPL];
+        A xc[EPL];
 #pragma unroll
         for (int j = 0; j < EPL; ++j) {
-            off[j] = valid ? (int)((cur.k[j] >> cb) & omask) : 0x7fffffff;
-            if constexpr (COMP) s[j] = two_prod(widen(cur.v[j]), xc[j]);
-            else s[j] = A(widen(cur.v[j])) * xc[j];
+            A l = A(0);
+            if constexpr (LDS) l = xs[min(c.k[j] & cmask, kZero)];
+            xc[j] = (c.k[j] & kKeyCached) != 0 ? l : g[j];
+            off[j] = valid ? (int)((c.k[j] >> cb) & omask) : 0x7fffffff;
+            if constexpr (COMP) s[j] = two_prod(widen(c.v[j]), xc[j]);
+            else s[j] = A(widen(c.v[j])) * xc[j];
         }
         if constexpr (SDDMM) {
             // x of each entry's row: a chunk's rows are few and consecutive,
             // so lane l loads x[r0 + l] once and each entry takes its row's
             // value by a lane shuffle; rows 64 or more past r0 load directly
-            const uint32_t r0 = it.row_base + (uint32_t)__builtin_amdgcn_readfirstlane(off[0]);
-            const A xw = a.xrow[min(r0 + (uint32_t)lane, (uint32_t)a.m - 1u)];
+            const uint32_t rr = it.row_base + (uint32_t)__builtin_amdgcn_readfirstlane(off[0]);
+            const A xw = a.xrow[min(rr + (uint32_t)lane, (uint32_t)a.m - 1u)];
             A dv[EPL];
             bool far = false;
 #pragma unroll
             for (int j = 0; j < EPL; ++j) {
-                const uint32_t d = it.row_base + (uint32_t)off[j] - r0;
+                const uint32_t d = it.row_base + (uint32_t)off[j] - rr;
                 const bool near = valid && d < 64u;
                 far = far || (valid && !near);
                 dv[j] = xc[j] * __shfl(xw, near ? (int)d : 0, kWave);
@@ -241,29 +291,34 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t 
             if (__any(far)) {
 #pragma unroll
                 for (int j = 0; j < EPL; ++j) {
-                    const uint32_t d = it.row_base + (uint32_t)off[j] - r0;
+                    const uint32_t d = it.row_base + (uint32_t)off[j] - rr;
                     if (valid && d >= 64u) dv[j] = xc[j] * a.xrow[min(it.row_base + (uint32_t)off[j], (uint32_t)a.m - 1u)];
                 }
             }
-            if (valid) {
-                u32x4* dst = reinterpret_cast<u32x4*>(a.dvals + e0 + (uint32_t)lane * EPL);
+            const rsrc_t rd = make_rsrc(a.dvals + it.e_begin, ne * (uint32_t)sizeof(A));
+            const uint32_t at = valid ? (r0 + (uint32_t)lane * EPL) * (uint32_t)sizeof(A) : kOutOfRange;
 #pragma unroll
-                for (int q = 0; q < EPL * (int)sizeof(A) / 16; ++q) {
-                    u32x4 w;
-                    __builtin_memcpy(&w, reinterpret_cast<const unsigned char*>(dv) + 16 * q, 16);
-                    __builtin_nontemporal_store(w, dst + q);
-                }
+            for (int q = 0; q < EPL * (int)sizeof(A) / 16; ++q) {
+                u32x4 w;
+                __builtin_memcpy(&w, reinterpret_cast<const unsigned char*>(dv) + 16 * q, 16);
+                __builtin_amdgcn_raw_buffer_store_b128(w, rd, at + 16 * q, 0, kNonTemporal);
             }
         }
+        // a finished run inside the loop: to y, or held when it is the first
+        // piece of a cut row
+        auto emit = [&](bool go, int o, Acc v) {
+            const bool to_partial = first_piece && o == 0;
+            if (go && to_partial) {
+                held = v;
+                has_held = true;
+            }
+            buffer_store<A>(finish(v), ry, go && !to_partial ? (uint32_t)o * (uint32_t)sizeof(A) : kOutOfRange);
+        };
         // the run left open by the previous chunk
-        if (carry_off >= 0) {
-            const int o0 = __builtin_amdgcn_readfirstlane(off[0]);
-            if (o0 == carry_off) {
-                if (lane == 0) s[0] += carry;
-            } else {
-                emit(lane == 0, carry_off, carry, false);
-            }
-        }
+        const int o0 = __builtin_amdgcn_readfirstlane(off[0]);
+        const bool cont = carry_off >= 0 && o0 == carry_off;
+        if (lane == 0 && cont) s[0] += carry;
+        emit(lane == 0 && carry_off >= 0 && !cont, carry_off, carry);
         // runs inside the lane: s[j] becomes the partial sum of the run
         // through entry j; ends[j] marks a run that stops at entry j
         bool ends[EPL];
@@ -283,18 +338,57 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t 
         for (int j = 0; j < EPL - 1; ++j) {
             Acc v = s[j];
             if (first && prev_off == off[0]) v += prev_T;
-            emit(valid && ends[j], off[j], v, false);
+            emit(valid && ends[j], off[j], v);
             first = first && !ends[j];
         }
-        emit(valid && lane < L && next_off != off[EPL - 1], off[EPL - 1], T, false);
+        emit(valid && lane < L && next_off != off[EPL - 1], off[EPL - 1], T);
         carry = read_lane(T, L);
         carry_off = __builtin_amdgcn_readlane(off[EPL - 1], L);
-        if (more) {
-            cur = nxt;
-            gather(cur.k, xc);
+    };
+
+    LaneChunk<V, EPL> c[RING];
+    A g[RING][EPL];
+#pragma unroll
+    for (int d = 0; d < RING - 1; ++d) c[d].load(rs, d * kChunk, ne, lane);
+    gather(c[0], g[0], true);
+    uint32_t r0 = 0;
+    for (bool more = true; more;) {
+#pragma unroll
+        for (int u = 0; u < RING; ++u) {
+            if (r0 >= ne) {
+                more = false;
+                break;
+            }
+            gather(c[(u + 1) % RING], g[(u + 1) % RING], r0 + kChunk < ne);
+            c[(u + RING - 1) % RING].load(rs, r0 + (RING - 1) * kChunk, ne, lane);
+            reduce(c[u], g[u], r0);
+            r0 += kChunk;
         }
     }
-    emit(lane == 0, carry_off, carry, true);
+    // the first piece of a cut row, if it ended inside the loop
+    if (first_piece) {
+        const uint64_t hm = __ballot(has_held);
+        if (hm != 0) {
+            const Acc v = read_lane(held, (int)__builtin_ctzll(hm));
+            if (lane == 0) {
+                const StreamSplit sp = a.splits[split_first];
+                put_partial(sp.partial_base + (item - sp.first_item), v);
+            }
+        }
+    }
+    // the last run: a piece of a cut row (first and last piece when the item
+    // holds one run), or a whole row
+    if (lane == 0 && carry_off >= 0) {
+        if (first_piece && carry_off == 0) {
+            const StreamSplit sp = a.splits[split_first];
+            put_partial(sp.partial_base + (item - sp.first_item), carry);
+        } else if (it.last_partial) {
+            const StreamSplit sp = a.splits[split_last];
+            put_partial(sp.partial_base + (item - sp.first_item), carry);
+        } else {
+            yout[it.row_base + (uint32_t)carry_off] = finish(carry);
+        }
+    }
 
     // ---- pieces of a cut row: the last piece to arrive sums them in order
     for (int which = 0; which < 2; ++which) {
@@ -314,10 +408,44 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t 
                     if constexpr (COMP) sum += f2{load_partial(a.partial + slot), load_partial(a.partial + slot + 1)};
                     else sum += load_partial(a.partial + slot);
                 }
-                a.y[sp.row] = finish(sum);
+                yout[sp.row] = finish(sum);
                 atomicExch(a.counters + sid, 0u);
             }
         }
+    }
+}
+
+// Column panels: after its items, a tile adds to y, for each of its rows
+// with panel entries, the row's panel sums (in panel order; written by the
+// panel kernel that ran before this launch).  Rows without panel entries are
+// not touched: a row cut into pieces, whose last piece may be stored by
+// another tile, never has any.  Every thread of the workgroup calls this.
+template <typename V, typename A>
+__device__ __forceinline__ void add_panel_sums(const StreamArgs<V, A>& a, uint32_t g) {
+    __syncthreads();   // the tile's y rows are stored
+    const uint32_t lo = a.tile_rows[2 * g], hi = a.tile_rows[2 * g + 1];
+    for (uint32_t r = lo + threadIdx.x; r <= hi; r += blockDim.x) {
+        const uint32_t mask = a.panel_mask[r];
+        if (mask == 0) continue;
+        A t = A(0);
+        for (uint32_t q = 0; q < a.panels; ++q)
+            if ((mask >> q) & 1u) t += a.panel_out[(size_t)q * (uint32_t)a.m + r];
+        a.y[r] = a.y[r] + t;
+    }
+}
+
+// One tile of the stream: stage its hot x lines in LDS, then each of its
+// waves streams one item, then (column panels) the tile adds its rows' panel
+// sums.  Every thread of the workgroup calls this.
+template <typename V, typename A, bool COMP, int EPL, bool SDDMM>
+__device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t tile, A* xs) {
+    stage_tile(a, tile, xs);
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // wave-uniform
+    const uint32_t item = tile * a.tile_items + wave;
+    if (wave < a.tile_items && item < a.n_items) stream_item<V, A, COMP, EPL, SDDMM, true>(a, item, xs, a.y);
+    if (a.panels > 1) {
+        const uint32_t g = a.tile_group[tile];
+        if (g != 0xffffffffu) add_panel_sums(a, g);
     }
 }
 
@@ -334,7 +462,7 @@ __device__ __forceinline__ void stream_tile_multi(const StreamArgs<V, A>& a, uin
     using Acc = typename std::conditional<COMP, f2, A>::type;
     const uint32_t kZero = a.tile_slots - 1;
     const int lane = lane_id();
-    const int wave = (int)(threadIdx.x >> 6);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // wave-uniform
     {
         const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1];
         for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) lines[s] = a.tile_lines[lb + s];
@@ -547,6 +675,26 @@ __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan
     }
 }
 
+
+// Column-panel units: workgroup b runs on XCD b % 8 (the hardware deals
+// workgroups round-robin over the XCDs), which serves panel (b % 8) % P; the
+// units of a panel are dealt over its 8 / P XCDs.  The waves of a unit take
+// its items in turn (no LDS: every key is a column or padding) and write the
+// rows' partial sums to panel q's output.  Panel items never cut rows.
+template <typename V, typename A, int EPL, bool SDDMM>
+__global__ __launch_bounds__(kPanelThreads) void stream_panel(StreamArgs<V, A> a) {
+    const uint32_t P = a.panels, per = kXcds / P;
+    const uint32_t xcd = blockIdx.x % kXcds, q = xcd % P;
+    const uint32_t u = a.unit_ptr[q] + (blockIdx.x / kXcds) * per + xcd / P;
+    if (u >= a.unit_ptr[q + 1]) return;
+    const uint32_t ib = a.units[4 * u], ie = a.units[4 * u + 1];
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint32_t waves = blockDim.x >> 6;
+    A* out = a.panel_out + (size_t)q * (uint32_t)a.m;
+    for (uint32_t item = ib + wave; item < ie; item += waves)
+        stream_item<V, A, false, EPL, SDDMM, false>(a, item, nullptr, out);
+}
+
 }  // namespace dev
 
 template <typename V, typename A, bool COMP, bool SDDMM>
@@ -556,7 +704,7 @@ hipError_t launch_stream_t(const StreamArgs<V, A>& a, int grid, hipStream_t s) {
     const size_t lds = (size_t)a.tile_slots * sizeof(A);
     static bool attr_set = false;   // allow more than 64 KB of dynamic LDS
     if (!attr_set) {
-        const hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        const hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxTileBytes);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
@@ -577,7 +725,7 @@ hipError_t launch_batch_t(const BatchPlan<V, A>* plans, uint32_t nplans, uint32_
     auto kern = dev::stream_batch<V, A, COMP, EPL>;
     static bool attr_set = false;
     if (!attr_set) {
-        const hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        const hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxTileBytes);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
@@ -613,6 +761,23 @@ hipError_t launch_batch(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uin
                         size_t lds, bool comp, hipStream_t s) {
     return comp ? launch_batch_t<dev::bf16_t, float, true>(p, np, units, grid, threads, lds, s)
                 : launch_batch_t<dev::bf16_t, float, false>(p, np, units, grid, threads, lds, s);
+}
+
+template <typename V, typename A>
+hipError_t launch_panels_t(const StreamArgs<V, A>& a, int grid, bool sddmm, hipStream_t s) {
+    constexpr int EPL = sizeof(V) >= 8 ? 4 : 8;
+    if (sddmm) hipLaunchKernelGGL((dev::stream_panel<V, A, EPL, true>), dim3(grid), dim3(kPanelThreads), 0, s, a);
+    else hipLaunchKernelGGL((dev::stream_panel<V, A, EPL, false>), dim3(grid), dim3(kPanelThreads), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_panels(const StreamArgs<double, double>& a, int grid, bool sddmm, hipStream_t s) {
+    return launch_panels_t(a, grid, sddmm, s);
+}
+hipError_t launch_panels(const StreamArgs<float, float>& a, int grid, bool sddmm, hipStream_t s) {
+    return launch_panels_t(a, grid, sddmm, s);
+}
+hipError_t launch_panels(const StreamArgs<dev::bf16_t, float>& a, int grid, bool sddmm, hipStream_t s) {
+    return launch_panels_t(a, grid, sddmm, s);
 }
 
 hipError_t launch_stream(const StreamArgs<double, double>& a, int grid, bool sddmm, hipStream_t s) {

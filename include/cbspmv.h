@@ -75,9 +75,10 @@ typedef struct cbspmv_opts {
     int schedule;         /* enum cbspmv_schedule                                    */
     int stream_item_nnz;  /* stream schedule: target entries per wave item (0=auto)  */
     int stream_tile_waves;/* stream schedule: waves per workgroup, 1..16 (0=auto: 8)   */
-    int stream_tile_kib;  /* stream schedule: LDS x cache per workgroup, KiB (0=96)  */
+    int stream_tile_kib;  /* stream schedule: LDS x cache per workgroup, 4..156 KiB (0=96) */
     int backward_mode;    /* enum cbspmv_backward_mode (cbspmv_backward)              */
-    int reserved[2];
+    int stream_panels;    /* stream schedule: x column panels 1, 2, 4 or 8 (0 = 1)    */
+    int reserved[1];
 } cbspmv_opts;
 
 /* Builder statistics: the GLOBAL_BLOCK scalars (format.h:63-77) plus counts. */
@@ -128,6 +129,12 @@ int cbspmv_plan_stats(const cbspmv_plan* plan, cbspmv_stats* stats);
  *   pl_idx (int32) pl_val (f64): payload streams in final block order
  *   image (u8): canonical packed slab bytes up to the y region, block
  *               addresses stored as byte offsets (0 = none), x region zero
+ * Stream schedule (u32 unless marked): stream_key stream_val(f64)
+ *   stream_slab(u8) stream_items (4 per item) stream_tiles stream_tile_lines
+ *   stream_empty_rows stream_value_pos(i64) stream_col_bits(i32)
+ *   stream_panels (i32: panels, tile item count) stream_units (4 per unit)
+ *   stream_unit_ptr stream_tile_rows (2 per tile with panel rows) stream_tile_group
+ *   stream_panel_mask(u8)
  * Returns CBSPMV_ERR_ARG for an unknown name. */
 int cbspmv_export(const cbspmv_plan* plan, const char* name, void* dst, size_t* count);
 /* Byte offset of a slab region in the canonical image ("total" = size). */
