@@ -29,10 +29,7 @@ _EXPORT_DTYPES = {
     "stream_tiles": np.uint32,
     "stream_tile_lines": np.uint32,
     "stream_empty_rows": np.uint32,
-    "stream_units": np.uint32,
-    "stream_unit_ptr": np.uint32,
-    "stream_tile_rows": np.uint32,
-    "stream_tile_group": np.uint32,
+    "stream_panel_items": np.uint32,
     "stream_panel_mask": np.uint8,
     "grad_slot": np.uint32,
     "stream_value_pos": np.int64,

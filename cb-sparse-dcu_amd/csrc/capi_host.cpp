@@ -60,6 +60,10 @@ int build_plan_object(const cbspmv_opts& o, int m, int n, int nnz, const int* ro
     if (o.stream_tile_waves) h.o.tile_items = o.stream_tile_waves;
     if (o.stream_tile_kib) h.o.tile_bytes = o.stream_tile_kib * 1024;
     h.o.panels = o.stream_panels;
+    if (h.o.panels > 1 && h.o.tile_items > kMaxTileItems / 2) {
+        delete p;
+        return fail(CBSPMV_ERR_ARG, "stream_panels > 1 needs stream_tile_waves <= 8 (a workgroup adds as many panel waves)");
+    }
     if (h.o.tile_items < 1 || h.o.tile_items > kMaxTileItems || h.o.tile_bytes < 4096 || h.o.tile_bytes > kMaxTileBytes) {
         delete p;
         return fail(CBSPMV_ERR_ARG, "stream tile shape out of range (1..16 waves, 4..156 KiB)");

@@ -87,16 +87,14 @@ inline int stream_val_bytes(int dtype) { return dtype == CBSPMV_F64 ? 8 : dtype 
 
 // Column panels.  When x is larger than one XCD's L2 can hold, the entries a
 // tile cannot serve from its LDS x cache (scattered columns) are moved to
-// panel items: panel q holds columns [q w, (q + 1) w), and the workgroups of
-// panel q's units run on the XCDs q, q + P, ... only, so each XCD gathers
-// from a w-column slice of x that stays in its L2.  A unit is the panel-q
-// entries of one tile's rows; it writes per-row partial sums to row r of
-// panel q's output vector.  The panel kernel runs first; each tile then adds
-// the panel sums of its rows to y once its own items are stored.
+// panel items: panel q holds columns [q w, (q + 1) w) in row order, and only
+// waves on the XCDs q, q + P, ... run panel q's items, so each XCD gathers
+// from a w-column slice of x that stays in its L2.  A panel item writes its
+// rows' partial sums to row r of panel q's output vector; a second launch
+// adds them to y.
 constexpr int kXcds = 8;                        // MI355X XCDs (workgroups dealt round-robin)
 constexpr int64_t kPanelL2Bytes = 4ll << 20;    // x up to this size is gathered whole
 constexpr int64_t kPanelBytes = 2ll << 20;      // panel width target (bytes of x)
-constexpr int kPanelWaves = 8;                  // waves (and items) per panel unit
 // stream_panels = 0 picks panels by the size of x only when this is set
 constexpr bool kPanelsByDefault = false;
 
@@ -167,14 +165,11 @@ struct Plan {
     std::vector<uint32_t> tile_lines;       // cached x line ids (element / 16)
     uint32_t partial_slots = 0;
     // column panels (panels > 1): items [0, tile_items_count) belong to the
-    // tiles, the rest to panel units
+    // tiles, the rest to the panels
     int panels = 1;
     uint32_t tile_items_count = 0;
-    std::vector<uint32_t> units;       // per unit: item begin, item end, panel, tile
-    std::vector<uint32_t> unit_ptr;    // panel q's units: [unit_ptr[q], unit_ptr[q + 1])
-    std::vector<uint32_t> tile_rows;   // per tile with panel rows: first row, last row
-    std::vector<uint32_t> tile_group;  // per tile: its tile_rows pair or ~0
-    std::vector<uint8_t> panel_mask;   // per row: panels holding entries of the row
+    std::vector<uint32_t> panel_item_ptr;  // panel q's items: [ptr[q], ptr[q + 1])
+    std::vector<uint8_t> panel_mask;       // per row: panels holding entries of the row
 };
 
 // Builder (builder.cpp): bit-exact with coo2block_gather / coo2block_comp.

@@ -30,7 +30,7 @@ struct DevicePlan {
     DeviceBuf key, val, items, item_split, item_split_first, splits,
         partial, counters, empty, tiles, tile_lines;
     // column panels
-    DeviceBuf units, unit_ptr, tile_rows, tile_group, panel_mask, panel_out;
+    DeviceBuf panel_items, panel_mask, panel_out;
     // value replacement: per stream entry holding input entries, its value's
     // byte offset in the stream slab and its input entries (CSR, input order)
     DeviceBuf sv_pos, sv_ptr, sv_src;

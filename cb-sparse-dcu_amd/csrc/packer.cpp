@@ -377,7 +377,7 @@ void layout_tiles(Plan& p, const StreamShape& sh, const Entries& E, std::vector<
 
 int stream_panels(const Plan& p) {
     if (p.o.panels > 0) return p.o.panels;
-    if (!kPanelsByDefault || p.o.accum == CBSPMV_ACC_COMPENSATED) return 1;
+    if (!kPanelsByDefault || p.o.accum == CBSPMV_ACC_COMPENSATED || p.o.tile_items > kMaxTileItems / 2) return 1;
     const int64_t x_bytes = (int64_t)p.n * (p.o.dtype == CBSPMV_F64 ? 8 : 4);
     if (x_bytes <= kPanelL2Bytes) return 1;
     int P = 2;
@@ -495,14 +495,17 @@ void build_stream(Plan& p) {
     layout_tiles(p, sh, E, p_col, p_off, p_zero, src);
 
     // ---- column panels: the entries whose x line is not staged in LDS are
-    // moved to panel items, grouped by (tile, panel), so that the workgroups
-    // of one XCD gather x from one panel of columns only (held in that XCD's
-    // L2).  Rows cut into pieces keep all their entries in the tiles.  A row
-    // left without tile entries keeps a zero entry there, so its tile stores
-    // its y before adding the panel sums.
+    // moved to panel items, one row-ordered stream per panel of columns, so
+    // that the waves of one XCD gather x from one panel only (held in that
+    // XCD's L2).  Every workgroup runs a tile's items and, on its other
+    // waves, items of its XCD's panel; the panel sums reach y in a second,
+    // small launch.  Rows cut into pieces keep all their entries in the
+    // tiles.  A row left without tile entries keeps a zero entry there, so
+    // that its tile stores its y before the panel sums are added.
     p.panels = 1;
     p.tile_items_count = (uint32_t)p.items.size();
-    p.units.clear(); p.unit_ptr.clear(); p.tile_rows.clear(); p.tile_group.clear(); p.panel_mask.clear();
+    p.panel_item_ptr.clear();
+    p.panel_mask.clear();
     const int P = stream_panels(p);
     if (P > 1) {
         const int64_t padded = (int64_t)p_col.size();
@@ -517,15 +520,15 @@ void build_stream(Plan& p) {
         }
         if (n_out > 0) {
             const uint32_t width = (uint32_t)(((int64_t)p.n + P - 1) / P + kLineElems - 1) / kLineElems * kLineElems;
-            Entries T, U;   // tile entries, panel entries
+            Entries T;                 // tile entries
+            std::vector<Entries> U(P); // panel entries, row order
             T.reserve((size_t)(total - n_out + 64));
-            U.reserve((size_t)n_out);
             for (int64_t r0 = 0; r0 < total;) {
                 int64_t r1 = r0 + 1;
                 while (r1 < total && E.row[r1] == E.row[r0]) ++r1;
                 bool kept = false;
                 for (int64_t q = r0; q < r1; ++q) {
-                    if (uncached[q]) U.push(E.row[q], E.col[q], E.val[q], 0);
+                    if (uncached[q]) U[E.col[q] / width].push(E.row[q], E.col[q], E.val[q], 0);
                     else { T.push(E.row[q], E.col[q], E.val[q], E.zero[q]); kept = true; }
                 }
                 if (!kept) T.push(E.row[r0], 0, 0.0, 1);
@@ -535,71 +538,26 @@ void build_stream(Plan& p) {
             layout_tiles(p, sh, T, p_col, p_off, p_zero, src);
             p.panels = P;
             p.tile_items_count = (uint32_t)p.items.size();
-            // tile of every row (rows of U always have tile entries)
-            const size_t NT = p.tiles.size() / 2;
-            std::vector<uint32_t> row_tile(p.m, ~0u);
-            for (size_t i = 0; i < p.items.size(); ++i) {
-                const StreamItem& it = p.items[i];
-                for (uint32_t e = it.e_begin; e < it.e_end; ++e) {
-                    const uint32_t r = it.row_base + p_off[e];
-                    if (row_tile[r] == ~0u) row_tile[r] = (uint32_t)(i / sh.tile_items);
-                }
-            }
-            // U in (tile, panel) order, rows in order inside (a stable counting sort)
-            const int64_t nu = U.size();
-            std::vector<int64_t> key_ptr(NT * P + 1, 0);
-            std::vector<uint32_t> ukey(nu);
-            for (int64_t q = 0; q < nu; ++q) {
-                ukey[q] = row_tile[U.row[q]] * (uint32_t)P + U.col[q] / width;
-                key_ptr[ukey[q] + 1]++;
-            }
-            for (size_t k = 0; k < NT * P; ++k) key_ptr[k + 1] += key_ptr[k];
-            Entries S;
-            S.row.resize(nu); S.col.resize(nu); S.val.resize(nu); S.zero.assign(nu, 0);
-            {
-                std::vector<int64_t> at(key_ptr.begin(), key_ptr.end() - 1);
-                for (int64_t q = 0; q < nu; ++q) {
-                    const int64_t d = at[ukey[q]]++;
-                    S.row[d] = U.row[q]; S.col[d] = U.col[q]; S.val[d] = U.val[q];
-                }
-            }
-            // units, dealt by panel: unit_ptr[q] .. unit_ptr[q + 1] are panel q's
             p.panel_mask.assign(p.m, 0);
-            for (int64_t q = 0; q < nu; ++q) p.panel_mask[S.row[q]] |= (uint8_t)(1u << (S.col[q] / width));
-            p.tile_group.assign(NT, ~0u);
-            for (size_t t = 0; t < NT; ++t) {
-                if (key_ptr[t * P + P] == key_ptr[t * P]) continue;
-                p.tile_group[t] = (uint32_t)(p.tile_rows.size() / 2);
-                const StreamItem& first = p.items[t * sh.tile_items];
-                const size_t last_i = std::min(p.items.size(), (t + 1) * sh.tile_items) - 1;
-                const StreamItem& last = p.items[last_i];
-                p.tile_rows.push_back(first.row_base);
-                p.tile_rows.push_back(last.row_base + p_off[last.e_end - 1]);
-            }
-            p.unit_ptr.assign(P + 1, 0);
+            // panel waves per panel: the workgroups on its XCDs times the
+            // panel waves of each (as many as the tile's)
+            const int64_t NT = (int64_t)(p.tiles.size() / 2);
+            const int64_t grid = std::max<int64_t>(kXcds, (NT + kXcds - 1) / kXcds * kXcds);
+            const int64_t waves = grid / P * (int64_t)sh.tile_items;
+            p.panel_item_ptr.assign(P + 1, (uint32_t)p.items.size());
             std::vector<Piece> pieces;
             std::vector<StreamSplit> no_splits;
             uint32_t no_slots = 0;
             for (int q = 0; q < P; ++q) {
-                for (size_t t = 0; t < NT; ++t) {
-                    const int64_t b = key_ptr[t * P + q], e = key_ptr[t * P + q + 1];
-                    if (b == e) continue;
-                    // at most kPanelWaves items: one per wave of the unit
-                    int64_t target = std::max<int64_t>(256, (e - b + kPanelWaves - 1) / kPanelWaves);
-                    for (;;) {
-                        pieces.clear();
-                        cut_items(S, b, e, target, false, sh.omask, 0, pieces, no_splits, no_slots);
-                        if (pieces.size() <= (size_t)kPanelWaves || target >= e - b) break;
-                        target += target / 8 + 1;
-                    }
-                    const uint32_t ib = (uint32_t)p.items.size();
-                    append_items(p, S, pieces, epl, p_col, p_off, p_zero, nullptr);
-                    p.units.push_back(ib);
-                    p.units.push_back((uint32_t)p.items.size());
-                    p.units.push_back((uint32_t)q);
-                    p.units.push_back((uint32_t)t);
+                const Entries& Uq = U[q];
+                for (int64_t k = 0; k < Uq.size(); ++k) p.panel_mask[Uq.row[k]] |= (uint8_t)(1u << q);
+                if (Uq.size() > 0) {
+                    const int64_t target = std::max<int64_t>(256, (Uq.size() + waves - 1) / waves);
+                    pieces.clear();
+                    cut_items(Uq, 0, Uq.size(), target, false, sh.omask, 0, pieces, no_splits, no_slots);
+                    append_items(p, Uq, pieces, epl, p_col, p_off, p_zero, nullptr);
                 }
-                p.unit_ptr[q + 1] = (uint32_t)(p.units.size() / 4);
+                p.panel_item_ptr[q + 1] = (uint32_t)p.items.size();
             }
             // panel keys: column (never staged), padding as in the tiles
             const size_t start = p.s_key.size();
@@ -801,10 +759,7 @@ long export_array(const Plan& p, const std::string& s, void* dst, size_t cap, in
     }
     if (s == "stream_col_bits") return copy_out(std::vector<int>{p.col_bits}, dst, cap);
     if (s == "stream_panels") return copy_out(std::vector<int>{p.panels, (int)p.tile_items_count}, dst, cap);
-    if (s == "stream_units") return copy_out(p.units, dst, cap);
-    if (s == "stream_unit_ptr") return copy_out(p.unit_ptr, dst, cap);
-    if (s == "stream_tile_rows") return copy_out(p.tile_rows, dst, cap);
-    if (s == "stream_tile_group") return copy_out(p.tile_group, dst, cap);
+    if (s == "stream_panel_items") return copy_out(p.panel_item_ptr, dst, cap);
     if (s == "stream_panel_mask") return copy_out(p.panel_mask, dst, cap);
     if (s == "plan_dtype") return copy_out(std::vector<int>{p.o.dtype}, dst, cap);
     if (s == "stream_value_pos") {   // byte offset in stream_slab of each entry's value

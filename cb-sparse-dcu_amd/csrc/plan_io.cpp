@@ -85,7 +85,7 @@ void members(IO& io, Plan& p, cbspmv_opts& o) {
     io.vec(p.item_split); io.vec(p.item_split_first); io.vec(p.splits); io.vec(p.empty_rows);
     io.vec(p.tiles); io.vec(p.tile_lines); io.pod(p.partial_slots);
     io.pod(p.panels); io.pod(p.tile_items_count);
-    io.vec(p.units); io.vec(p.unit_ptr); io.vec(p.tile_rows); io.vec(p.tile_group); io.vec(p.panel_mask);
+    io.vec(p.panel_item_ptr); io.vec(p.panel_mask);
 }
 
 void write_offsets(Writer& w, const Plan& p) {

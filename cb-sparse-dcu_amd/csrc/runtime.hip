@@ -40,7 +40,7 @@ void release_device(DevicePlan& d) {
     for (DeviceBuf* b : {&d.slab, &d.zero_strips, &d.key, &d.val, &d.items, &d.item_split,
                          &d.item_split_first, &d.splits, &d.partial, &d.counters, &d.empty, &d.tiles, &d.tile_lines,
                          &d.sv_pos, &d.sv_ptr, &d.sv_src, &d.grad_slot, &d.dvals_stored, &d.in_row, &d.in_col,
-                         &d.units, &d.unit_ptr, &d.tile_rows, &d.tile_group, &d.panel_mask, &d.panel_out})
+                         &d.panel_items, &d.panel_mask, &d.panel_out})
         release(*b);
     d.uploaded = false;
 }
@@ -102,10 +102,7 @@ int upload_plan(cbspmv_plan& p, hipStream_t s) {
     if ((rc = upload_vec(d.tiles, h.tiles, s))) return rc;
     if ((rc = upload_vec(d.tile_lines, h.tile_lines, s))) return rc;
     if (h.panels > 1) {
-        if ((rc = upload_vec(d.units, h.units, s))) return rc;
-        if ((rc = upload_vec(d.unit_ptr, h.unit_ptr, s))) return rc;
-        if ((rc = upload_vec(d.tile_rows, h.tile_rows, s))) return rc;
-        if ((rc = upload_vec(d.tile_group, h.tile_group, s))) return rc;
+        if ((rc = upload_vec(d.panel_items, h.panel_item_ptr, s))) return rc;
         if ((rc = upload_vec(d.panel_mask, h.panel_mask, s))) return rc;
         if ((rc = upload_bytes(d.panel_out, nullptr, (size_t)h.panels * h.m * vec_elem_bytes(h), s))) return rc;
     }
@@ -117,18 +114,6 @@ int upload_plan(cbspmv_plan& p, hipStream_t s) {
 int stream_grid(const Plan& h) {
     const long long tiles = (long long)h.tiles.size() / 2;
     return (int)std::max<long long>(8, (tiles + 7) / 8 * 8);
-}
-
-// Panel kernel grid: 8 workgroups per round, each XCD of panel q taking
-// its share of panel q's units.
-int panel_grid(const Plan& h) {
-    const uint32_t per = (uint32_t)(kXcds / h.panels);
-    uint32_t rounds = 1;
-    for (int q = 0; q < h.panels; ++q) {
-        const uint32_t nu = h.unit_ptr[q + 1] - h.unit_ptr[q];
-        rounds = std::max(rounds, (nu + per - 1) / per);
-    }
-    return (int)(rounds * kXcds);
 }
 
 // The stream kernels address x and y through buffer resources of at most
@@ -163,14 +148,10 @@ int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s, const void
         a.m = h.m;
         a.n = h.n;
         a.panels = (uint32_t)h.panels;
-        a.units = static_cast<const uint32_t*>(d.units.ptr);
-        a.unit_ptr = static_cast<const uint32_t*>(d.unit_ptr.ptr);
-        a.tile_rows = static_cast<const uint32_t*>(d.tile_rows.ptr);
-        a.tile_group = static_cast<const uint32_t*>(d.tile_group.ptr);
-        a.panel_mask = static_cast<const uint8_t*>(d.panel_mask.ptr);
+        a.panel_item_ptr = static_cast<const uint32_t*>(d.panel_items.ptr);
     };
     const bool panels = h.panels > 1;
-    const int pgrid = panels ? panel_grid(h) : 0;
+    const uint8_t* mask = static_cast<const uint8_t*>(d.panel_mask.ptr);
     if (h.o.dtype == CBSPMV_F64) {
         StreamArgs<double, double> a{};
         fill(a);
@@ -180,8 +161,8 @@ int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s, const void
         a.xrow = static_cast<const double*>(xrow);
         a.dvals = static_cast<double*>(dvals);
         a.panel_out = static_cast<double*>(d.panel_out.ptr);
-        e = panels ? launch_panels(a, pgrid, dvals != nullptr, s) : hipSuccess;
-        if (e == hipSuccess) e = launch_stream(a, grid, dvals != nullptr, s);
+        e = launch_stream(a, grid, dvals != nullptr, s);
+        if (e == hipSuccess && panels) e = launch_panel_sums(a.y, a.panel_out, mask, (uint32_t)h.m, a.panels, s);
     } else if (h.o.dtype == CBSPMV_F32) {
         StreamArgs<float, float> a{};
         fill(a);
@@ -191,8 +172,8 @@ int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s, const void
         a.xrow = static_cast<const float*>(xrow);
         a.dvals = static_cast<float*>(dvals);
         a.panel_out = static_cast<float*>(d.panel_out.ptr);
-        e = panels ? launch_panels(a, pgrid, dvals != nullptr, s) : hipSuccess;
-        if (e == hipSuccess) e = launch_stream(a, grid, h.o.accum == CBSPMV_ACC_COMPENSATED, dvals != nullptr, s);
+        e = launch_stream(a, grid, h.o.accum == CBSPMV_ACC_COMPENSATED, dvals != nullptr, s);
+        if (e == hipSuccess && panels) e = launch_panel_sums(a.y, a.panel_out, mask, (uint32_t)h.m, a.panels, s);
     } else {
         StreamArgs<dev::bf16_t, float> a{};
         fill(a);
@@ -202,8 +183,8 @@ int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s, const void
         a.xrow = static_cast<const float*>(xrow);
         a.dvals = static_cast<float*>(dvals);
         a.panel_out = static_cast<float*>(d.panel_out.ptr);
-        e = panels ? launch_panels(a, pgrid, dvals != nullptr, s) : hipSuccess;
-        if (e == hipSuccess) e = launch_stream(a, grid, h.o.accum == CBSPMV_ACC_COMPENSATED, dvals != nullptr, s);
+        e = launch_stream(a, grid, h.o.accum == CBSPMV_ACC_COMPENSATED, dvals != nullptr, s);
+        if (e == hipSuccess && panels) e = launch_panel_sums(a.y, a.panel_out, mask, (uint32_t)h.m, a.panels, s);
     }
     if (e != hipSuccess) return hip_fail(e, "stream schedule launch");
     return CBSPMV_OK;

@@ -18,6 +18,7 @@
 // dx = A^T g it writes, for every stream entry (row j of A^T, column i),
 // dvals = g[i] * x[j] in stream order, reusing the g[i] the product gathers
 // anyway; lanes own consecutive entries, so those stores are coalesced.
+#include <algorithm>
 #include <type_traits>
 
 #include "kernels.hpp"
@@ -171,11 +172,12 @@ __device__ __forceinline__ f2 wave_shr1(f2 v) { return f2{dpp<kWaveShr1, 0xf>(v.
 
 // Stages a tile's hot x lines in LDS, PER slots per thread per pass with all
 // line-id loads, then all x loads, issued back to back (16 consecutive
-// threads read one line); the last slot stays 0 for the padding entries.
+// threads read one line): two dependent round trips per pass, one pass for
+// the default cache; the last slot stays 0 for the padding entries.
 // Every thread of the workgroup calls this; it ends with a workgroup barrier.
 template <typename V, typename A>
 __device__ __forceinline__ void stage_tile(const StreamArgs<V, A>& a, uint32_t tile, A* xs) {
-    constexpr int PER = 8;
+    constexpr int PER = 24;   // one pass for a 96 KB cache and 8 waves
     const uint32_t nthreads = blockDim.x;
     const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1] * kLineElems;
     for (uint32_t s0 = threadIdx.x; s0 < cnt; s0 += PER * nthreads) {
@@ -202,11 +204,199 @@ __device__ __forceinline__ void stage_tile(const StreamArgs<V, A>& a, uint32_t t
     __syncthreads();
 }
 
+// Pieces of a cut row: each piece's wave takes a ticket once its partial
+// sum is stored; the last piece to arrive sums them in piece order.
+template <typename V, typename A, bool COMP>
+__device__ __forceinline__ void join_pieces(const StreamArgs<V, A>& a, uint32_t split_first, uint32_t split_last,
+                                            A* yout) {
+    using Acc = typename std::conditional<COMP, f2, A>::type;
+    constexpr int kParts = COMP ? 2 : 1;
+    const int lane = lane_id();
+    for (int which = 0; which < 2; ++which) {
+        const uint32_t sid = which == 0 ? split_first : split_last;
+        if (sid == 0xffffffffu || (which == 1 && sid == split_first)) continue;
+        __threadfence();
+        uint32_t ticket = 0;
+        if (lane == 0) ticket = atomicAdd(a.counters + sid, 1u);
+        ticket = __shfl(ticket, 0, kWave);
+        const StreamSplit sp = a.splits[sid];
+        if (ticket == sp.pieces - 1) {
+            __threadfence();
+            if (lane == 0) {
+                Acc sum{};
+                for (uint32_t q = 0; q < sp.pieces; ++q) {
+                    const uint32_t slot = kParts * (sp.partial_base + q);
+                    if constexpr (COMP) sum += f2{load_partial(a.partial + slot), load_partial(a.partial + slot + 1)};
+                    else sum += load_partial(a.partial + slot);
+                }
+                yout[sp.row] = finish(sum);
+                atomicExch(a.counters + sid, 0u);
+            }
+        }
+    }
+}
+
+// Chunks in flight per wave (the ring of stream_item): 5 when a chunk holds
+// 12 registers (FP64, BF16), 4 when it holds 16 (FP32); 3 where registers
+// are short (the batched kernel, FP32 backward).  The ring pays off when x is
+// gathered from beyond L2; with x in L2 the one-ahead loop of
+// stream_item_near (ring size 2) is faster.
+template <typename V, int EPL>
+constexpr int deep_ring() { return EPL * (4 + (int)sizeof(V)) / 4 <= 12 ? 5 : 4; }
+constexpr int kNearRing = 2;
+
+// One item with x in L2: chunk c + 1's keys and values are loaded (non-
+// temporal) while chunk c is reduced, then chunk c + 1's x is gathered; lanes
+// without a gather or a finished run skip the instruction.
+template <typename V, typename A, bool COMP, int EPL, bool SDDMM, bool LDS>
+__device__ __forceinline__ void stream_item_near(const StreamArgs<V, A>& a, uint32_t item, const A* xs, A* yout) {
+    using Acc = typename std::conditional<COMP, f2, A>::type;
+    const uint32_t kZero = a.tile_slots - 1;
+    const int lane = lane_id();
+    const StreamItem it = a.items[item];
+    const int cb = a.col_bits;
+    const uint32_t cmask = (1u << cb) - 1u;
+    const uint32_t omask = (1u << (31 - cb)) - 1u;
+    const uint32_t split_first = it.first_partial ? a.item_split_first[item] : 0xffffffffu;
+    const uint32_t split_last = it.last_partial ? a.item_split[item] : 0xffffffffu;
+    auto put_partial = [&](uint32_t slot, Acc v) {
+        if constexpr (COMP) {
+            a.partial[2 * slot] = v.hi;
+            a.partial[2 * slot + 1] = v.lo;
+        } else {
+            a.partial[slot] = v;
+        }
+    };
+    // a finished run: to y, or to the partial buffer when it is a piece of
+    // a cut row (the item's first run, offset 0, or its last run)
+    auto emit = [&](bool go, int off, Acc v, bool final_run) {
+        if (!go) return;
+        if (it.first_partial && off == 0) {
+            const StreamSplit sp = a.splits[split_first];
+            put_partial(sp.partial_base + (item - sp.first_item), v);
+        } else if (final_run && it.last_partial) {
+            const StreamSplit sp = a.splits[split_last];
+            put_partial(sp.partial_base + (item - sp.first_item), v);
+        } else {
+            yout[it.row_base + (uint32_t)off] = finish(v);
+        }
+    };
+    auto gather = [&](const uint32_t* kk, A* xx) {
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+            const uint32_t c = kk[j] & cmask;
+            const bool cached = (kk[j] & kKeyCached) != 0;
+            A g = A(0);
+            if (!cached) g = a.x[c];
+            A l = A(0);
+            if constexpr (LDS) l = xs[min(c, kZero)];
+            xx[j] = cached ? l : g;
+        }
+    };
+    constexpr uint32_t kChunk = kWave * EPL;
+    auto lanes_of = [&](uint32_t e0) { return min(64u, (it.e_end - e0) / EPL); };
+
+    int carry_off = -1;
+    Acc carry{};
+    Chunk<V, EPL> cur, nxt;
+    A xc[EPL];
+    cur.load(a.slab, it.e_begin, lanes_of(it.e_begin), lane);
+    gather(cur.k, xc);
+    for (uint32_t e0 = it.e_begin; e0 < it.e_end; e0 += kChunk) {
+        const bool more = e0 + kChunk < it.e_end;
+        if (more) nxt.load(a.slab, e0 + kChunk, lanes_of(e0 + kChunk), lane);
+        const int L = (int)min(63u, (it.e_end - e0) / EPL - 1);   // last lane with entries
+        const bool valid = lane <= L;
+        int off[EPL];
+        Acc s[EPL];
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+            off[j] = valid ? (int)((cur.k[j] >> cb) & omask) : 0x7fffffff;
+            if constexpr (COMP) s[j] = two_prod(widen(cur.v[j]), xc[j]);
+            else s[j] = A(widen(cur.v[j])) * xc[j];
+        }
+        if constexpr (SDDMM) {
+            // x of each entry's row: lane l loads x[r0 + l] once and each
+            // entry takes its row's value by a lane shuffle; rows 64 or more
+            // past r0 load directly
+            const uint32_t r0 = it.row_base + (uint32_t)__builtin_amdgcn_readfirstlane(off[0]);
+            const A xw = a.xrow[min(r0 + (uint32_t)lane, (uint32_t)a.m - 1u)];
+            A dv[EPL];
+            bool far = false;
+#pragma unroll
+            for (int j = 0; j < EPL; ++j) {
+                const uint32_t d = it.row_base + (uint32_t)off[j] - r0;
+                const bool near = valid && d < 64u;
+                far = far || (valid && !near);
+                dv[j] = xc[j] * __shfl(xw, near ? (int)d : 0, kWave);
+            }
+            if (__any(far)) {
+#pragma unroll
+                for (int j = 0; j < EPL; ++j) {
+                    const uint32_t d = it.row_base + (uint32_t)off[j] - r0;
+                    if (valid && d >= 64u) dv[j] = xc[j] * a.xrow[min(it.row_base + (uint32_t)off[j], (uint32_t)a.m - 1u)];
+                }
+            }
+            if (valid) {
+                u32x4* dst = reinterpret_cast<u32x4*>(a.dvals + e0 + (uint32_t)lane * EPL);
+#pragma unroll
+                for (int q = 0; q < EPL * (int)sizeof(A) / 16; ++q) {
+                    u32x4 w;
+                    __builtin_memcpy(&w, reinterpret_cast<const unsigned char*>(dv) + 16 * q, 16);
+                    __builtin_nontemporal_store(w, dst + q);
+                }
+            }
+        }
+        // the run left open by the previous chunk
+        if (carry_off >= 0) {
+            const int o0 = __builtin_amdgcn_readfirstlane(off[0]);
+            if (o0 == carry_off) {
+                if (lane == 0) s[0] += carry;
+            } else {
+                emit(lane == 0, carry_off, carry, false);
+            }
+        }
+        // runs inside the lane, then across lanes (segmented scan)
+        bool ends[EPL];
+#pragma unroll
+        for (int j = 1; j < EPL; ++j) {
+            const bool same = off[j] == off[j - 1];
+            ends[j - 1] = !same;
+            if (same) s[j] += s[j - 1];
+        }
+        const Acc T = seg_scan_dpp(s[EPL - 1], off[EPL - 1]);
+        const int prev_off = wave_shift<kWaveShr1>(off[EPL - 1], -1);
+        const Acc prev_T = wave_shr1(T);
+        const int next_off = wave_shift<kWaveShl1>(off[0], -1);
+        bool first = true;   // still in the lane's first run
+#pragma unroll
+        for (int j = 0; j < EPL - 1; ++j) {
+            Acc v = s[j];
+            if (first && prev_off == off[0]) v += prev_T;
+            emit(valid && ends[j], off[j], v, false);
+            first = first && !ends[j];
+        }
+        emit(valid && lane < L && next_off != off[EPL - 1], off[EPL - 1], T, false);
+        carry = read_lane(T, L);
+        carry_off = __builtin_amdgcn_readlane(off[EPL - 1], L);
+        if (more) {
+            cur = nxt;
+            gather(cur.k, xc);
+        }
+    }
+    emit(lane == 0, carry_off, carry, true);
+    join_pieces<V, A, COMP>(a, split_first, split_last, yout);
+}
+
 // One item, streamed by one wave: y rows go to yout (y, or a panel's partial
 // sums).  With LDS, keys flagged as staged read x from the tile's LDS cache
 // xs; without (panel items), such keys are padding and read 0.
-template <typename V, typename A, bool COMP, int EPL, bool SDDMM, bool LDS>
+template <typename V, typename A, bool COMP, int EPL, bool SDDMM, bool LDS, int RING>
 __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t item, const A* xs, A* yout) {
+    if constexpr (RING == kNearRing) {
+        stream_item_near<V, A, COMP, EPL, SDDMM, LDS>(a, item, xs, yout);
+        return;
+    }
     using Acc = typename std::conditional<COMP, f2, A>::type;
     const uint32_t kZero = a.tile_slots - 1;
     const int lane = lane_id();
@@ -221,7 +411,6 @@ __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t 
     // when it is a piece of a cut row (the item's first run, offset 0, or its
     // last run).  Pieces keep their full accumulator (hi and lo of a
     // compensated sum).
-    constexpr int kParts = COMP ? 2 : 1;
     auto put_partial = [&](uint32_t slot, Acc v) {
         if constexpr (COMP) {
             a.partial[2 * slot] = v.hi;
@@ -235,10 +424,12 @@ __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t 
     // every pass (buffer instructions whose out-of-range lanes neither read nor
     // write), so the compiler can count them and wait for exactly the load a
     // value comes from: chunk c + RING - 1 is loaded and the x of chunk c + 1
-    // gathered while chunk c is reduced.
+    // gathered while chunk c is reduced.  The gather of chunk c + 1 waits for
+    // its keys, loaded RING - 2 chunks earlier: that distance, not the wave
+    // count, hides the HBM latency.
+    static_assert(RING == kNearRing || RING >= 3, "the ring holds the reduced, the gathered and a loading chunk");
     constexpr uint32_t EB = 4 + sizeof(V);
     constexpr uint32_t kChunk = kWave * EPL;
-    constexpr int RING = 3;
     const uint32_t ne = it.e_end - it.e_begin;
     const rsrc_t rs = make_rsrc(a.slab + (size_t)it.e_begin * EB, ne * EB);
     const rsrc_t rx = make_rsrc(a.x, (uint32_t)a.n * (uint32_t)sizeof(A));
@@ -390,63 +581,18 @@ __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t 
         }
     }
 
-    // ---- pieces of a cut row: the last piece to arrive sums them in order
-    for (int which = 0; which < 2; ++which) {
-        const uint32_t sid = which == 0 ? split_first : split_last;
-        if (sid == 0xffffffffu || (which == 1 && sid == split_first)) continue;
-        __threadfence();
-        uint32_t ticket = 0;
-        if (lane == 0) ticket = atomicAdd(a.counters + sid, 1u);
-        ticket = __shfl(ticket, 0, kWave);
-        const StreamSplit sp = a.splits[sid];
-        if (ticket == sp.pieces - 1) {
-            __threadfence();
-            if (lane == 0) {
-                Acc sum{};
-                for (uint32_t q = 0; q < sp.pieces; ++q) {
-                    const uint32_t slot = kParts * (sp.partial_base + q);
-                    if constexpr (COMP) sum += f2{load_partial(a.partial + slot), load_partial(a.partial + slot + 1)};
-                    else sum += load_partial(a.partial + slot);
-                }
-                yout[sp.row] = finish(sum);
-                atomicExch(a.counters + sid, 0u);
-            }
-        }
-    }
-}
-
-// Column panels: after its items, a tile adds to y, for each of its rows
-// with panel entries, the row's panel sums (in panel order; written by the
-// panel kernel that ran before this launch).  Rows without panel entries are
-// not touched: a row cut into pieces, whose last piece may be stored by
-// another tile, never has any.  Every thread of the workgroup calls this.
-template <typename V, typename A>
-__device__ __forceinline__ void add_panel_sums(const StreamArgs<V, A>& a, uint32_t g) {
-    __syncthreads();   // the tile's y rows are stored
-    const uint32_t lo = a.tile_rows[2 * g], hi = a.tile_rows[2 * g + 1];
-    for (uint32_t r = lo + threadIdx.x; r <= hi; r += blockDim.x) {
-        const uint32_t mask = a.panel_mask[r];
-        if (mask == 0) continue;
-        A t = A(0);
-        for (uint32_t q = 0; q < a.panels; ++q)
-            if ((mask >> q) & 1u) t += a.panel_out[(size_t)q * (uint32_t)a.m + r];
-        a.y[r] = a.y[r] + t;
-    }
+    join_pieces<V, A, COMP>(a, split_first, split_last, yout);
 }
 
 // One tile of the stream: stage its hot x lines in LDS, then each of its
-// waves streams one item, then (column panels) the tile adds its rows' panel
-// sums.  Every thread of the workgroup calls this.
-template <typename V, typename A, bool COMP, int EPL, bool SDDMM>
+// first tile_items waves streams one item.  Every thread of the workgroup
+// calls this.
+template <typename V, typename A, bool COMP, int EPL, bool SDDMM, int RING>
 __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t tile, A* xs) {
     stage_tile(a, tile, xs);
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // wave-uniform
     const uint32_t item = tile * a.tile_items + wave;
-    if (wave < a.tile_items && item < a.n_items) stream_item<V, A, COMP, EPL, SDDMM, true>(a, item, xs, a.y);
-    if (a.panels > 1) {
-        const uint32_t g = a.tile_group[tile];
-        if (g != 0xffffffffu) add_panel_sums(a, g);
-    }
+    if (wave < a.tile_items && item < a.n_items) stream_item<V, A, COMP, EPL, SDDMM, true, RING>(a, item, xs, a.y);
 }
 
 // Several right-hand sides at once: each chunk of keys and values is loaded
@@ -606,7 +752,7 @@ __device__ __forceinline__ void stream_tile_multi(const StreamArgs<V, A>& a, uin
     }
 }
 
-template <typename V, typename A, bool COMP, int EPL, bool SDDMM>
+template <typename V, typename A, bool COMP, int EPL, bool SDDMM, bool NEAR>
 __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     A* xs = reinterpret_cast<A*>(lds_raw);   // tile_slots elements (dynamic LDS)
@@ -618,8 +764,36 @@ __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A
     const uint32_t nthreads = blockDim.x;
     for (uint32_t t = blockIdx.x * nthreads + threadIdx.x; t < a.n_empty_rows; t += gridDim.x * nthreads)
         a.y[a.empty_rows[t]] = A(0);
-    if (tile >= a.n_tiles) return;
-    stream_tile<V, A, COMP, EPL, SDDMM>(a, tile, xs);
+    constexpr int RING = NEAR ? kNearRing : SDDMM && EPL == 8 ? 3 : deep_ring<V, EPL>();
+    if (tile < a.n_tiles) stream_tile<V, A, COMP, EPL, SDDMM, RING>(a, tile, xs);
+    if (a.panels <= 1) return;
+    // Column panels: waves [tile_items, 2 tile_items) run items of panel
+    // (XCD % P) (the grid is a multiple of 8: workgroup b is on XCD b % 8),
+    // dealt over the panel waves of the panel's XCDs, into panel q's sums.
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (wave < a.tile_items) return;
+    const uint32_t P = a.panels, per = kXcds / P;
+    const uint32_t xcd = blockIdx.x % kXcds, q = xcd % P;
+    const uint32_t rank = (blockIdx.x / kXcds) * per + xcd / P;   // among panel q's workgroups
+    const uint32_t stride = (gridDim.x / kXcds) * per * a.tile_items;
+    A* out = a.panel_out + (size_t)q * (uint32_t)a.m;
+    for (uint32_t i = a.panel_item_ptr[q] + rank * a.tile_items + (wave - a.tile_items); i < a.panel_item_ptr[q + 1];
+         i += stride)
+        stream_item<V, A, false, EPL, SDDMM, false, RING>(a, i, nullptr, out);
+}
+
+// y[r] += the panel sums of row r (in panel order), for rows with panel
+// entries; other rows are not touched.
+template <typename A>
+__global__ __launch_bounds__(256) void panel_sums(A* y, const A* out, const uint8_t* mask, uint32_t m, uint32_t P) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x) {
+        const uint32_t mk = mask[r];
+        if (mk == 0) continue;
+        A t = A(0);
+        for (uint32_t q = 0; q < P; ++q)
+            if ((mk >> q) & 1u) t += out[(size_t)q * m + r];
+        y[r] = y[r] + t;
+    }
 }
 
 // Batched persistent launch: work units are (plan, tile) pairs, dealt to
@@ -657,7 +831,7 @@ __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan
             a.y = bp.y;
             a.partial = bp.partial;
             a.counters = bp.counters;
-            stream_tile<V, A, COMP, EPL, false>(a, tile, xs);
+            stream_tile<V, A, COMP, EPL, false, kNearRing>(a, tile, xs);
             __syncthreads();   // LDS is restaged by the next unit
             continue;
         }
@@ -676,31 +850,12 @@ __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan
 }
 
 
-// Column-panel units: workgroup b runs on XCD b % 8 (the hardware deals
-// workgroups round-robin over the XCDs), which serves panel (b % 8) % P; the
-// units of a panel are dealt over its 8 / P XCDs.  The waves of a unit take
-// its items in turn (no LDS: every key is a column or padding) and write the
-// rows' partial sums to panel q's output.  Panel items never cut rows.
-template <typename V, typename A, int EPL, bool SDDMM>
-__global__ __launch_bounds__(kPanelThreads) void stream_panel(StreamArgs<V, A> a) {
-    const uint32_t P = a.panels, per = kXcds / P;
-    const uint32_t xcd = blockIdx.x % kXcds, q = xcd % P;
-    const uint32_t u = a.unit_ptr[q] + (blockIdx.x / kXcds) * per + xcd / P;
-    if (u >= a.unit_ptr[q + 1]) return;
-    const uint32_t ib = a.units[4 * u], ie = a.units[4 * u + 1];
-    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint32_t waves = blockDim.x >> 6;
-    A* out = a.panel_out + (size_t)q * (uint32_t)a.m;
-    for (uint32_t item = ib + wave; item < ie; item += waves)
-        stream_item<V, A, false, EPL, SDDMM, false>(a, item, nullptr, out);
-}
-
 }  // namespace dev
 
-template <typename V, typename A, bool COMP, bool SDDMM>
+template <typename V, typename A, bool COMP, bool SDDMM, bool NEAR>
 hipError_t launch_stream_t(const StreamArgs<V, A>& a, int grid, hipStream_t s) {
     constexpr int EPL = sizeof(V) >= 8 ? 4 : 8;
-    auto kern = dev::stream_spmv<V, A, COMP, EPL, SDDMM>;
+    auto kern = dev::stream_spmv<V, A, COMP, EPL, SDDMM, NEAR>;
     const size_t lds = (size_t)a.tile_slots * sizeof(A);
     static bool attr_set = false;   // allow more than 64 KB of dynamic LDS
     if (!attr_set) {
@@ -708,14 +863,22 @@ hipError_t launch_stream_t(const StreamArgs<V, A>& a, int grid, hipStream_t s) {
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * a.tile_items), lds, s, a);
+    // with column panels, as many panel waves as tile waves
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * a.tile_items * (a.panels > 1 ? 2 : 1)), lds, s, a);
     return hipGetLastError();
+}
+
+// x within one XCD's L2 (kPanelL2Bytes): the one-ahead loop, else the ring
+template <typename V, typename A, bool COMP, bool SDDMM>
+hipError_t launch_stream_x(const StreamArgs<V, A>& a, int grid, hipStream_t s) {
+    const bool near = (int64_t)a.n * (int64_t)sizeof(A) <= kPanelL2Bytes;
+    return near ? launch_stream_t<V, A, COMP, SDDMM, true>(a, grid, s) : launch_stream_t<V, A, COMP, SDDMM, false>(a, grid, s);
 }
 
 template <typename V, typename A>
 hipError_t launch_stream_any(const StreamArgs<V, A>& a, int grid, bool comp, bool sddmm, hipStream_t s) {
-    if (comp) return sddmm ? launch_stream_t<V, A, true, true>(a, grid, s) : launch_stream_t<V, A, true, false>(a, grid, s);
-    return sddmm ? launch_stream_t<V, A, false, true>(a, grid, s) : launch_stream_t<V, A, false, false>(a, grid, s);
+    if (comp) return sddmm ? launch_stream_x<V, A, true, true>(a, grid, s) : launch_stream_x<V, A, true, false>(a, grid, s);
+    return sddmm ? launch_stream_x<V, A, false, true>(a, grid, s) : launch_stream_x<V, A, false, false>(a, grid, s);
 }
 
 template <typename V, typename A, bool COMP>
@@ -763,26 +926,24 @@ hipError_t launch_batch(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uin
                 : launch_batch_t<dev::bf16_t, float, false>(p, np, units, grid, threads, lds, s);
 }
 
-template <typename V, typename A>
-hipError_t launch_panels_t(const StreamArgs<V, A>& a, int grid, bool sddmm, hipStream_t s) {
-    constexpr int EPL = sizeof(V) >= 8 ? 4 : 8;
-    if (sddmm) hipLaunchKernelGGL((dev::stream_panel<V, A, EPL, true>), dim3(grid), dim3(kPanelThreads), 0, s, a);
-    else hipLaunchKernelGGL((dev::stream_panel<V, A, EPL, false>), dim3(grid), dim3(kPanelThreads), 0, s, a);
+template <typename A>
+hipError_t launch_panel_sums_t(A* y, const A* out, const uint8_t* mask, uint32_t m, uint32_t panels, hipStream_t s) {
+    const uint32_t grid = std::min<uint32_t>(4096u, (m + 255u) / 256u);
+    hipLaunchKernelGGL(dev::panel_sums<A>, dim3(std::max(grid, 1u)), dim3(256), 0, s, y, out, mask, m, panels);
     return hipGetLastError();
 }
-hipError_t launch_panels(const StreamArgs<double, double>& a, int grid, bool sddmm, hipStream_t s) {
-    return launch_panels_t(a, grid, sddmm, s);
+hipError_t launch_panel_sums(double* y, const double* out, const uint8_t* mask, uint32_t m, uint32_t panels,
+                             hipStream_t s) {
+    return launch_panel_sums_t(y, out, mask, m, panels, s);
 }
-hipError_t launch_panels(const StreamArgs<float, float>& a, int grid, bool sddmm, hipStream_t s) {
-    return launch_panels_t(a, grid, sddmm, s);
-}
-hipError_t launch_panels(const StreamArgs<dev::bf16_t, float>& a, int grid, bool sddmm, hipStream_t s) {
-    return launch_panels_t(a, grid, sddmm, s);
+hipError_t launch_panel_sums(float* y, const float* out, const uint8_t* mask, uint32_t m, uint32_t panels,
+                             hipStream_t s) {
+    return launch_panel_sums_t(y, out, mask, m, panels, s);
 }
 
 hipError_t launch_stream(const StreamArgs<double, double>& a, int grid, bool sddmm, hipStream_t s) {
-    return sddmm ? launch_stream_t<double, double, false, true>(a, grid, s)
-                 : launch_stream_t<double, double, false, false>(a, grid, s);
+    return sddmm ? launch_stream_x<double, double, false, true>(a, grid, s)
+                 : launch_stream_x<double, double, false, false>(a, grid, s);
 }
 hipError_t launch_stream(const StreamArgs<float, float>& a, int grid, bool comp, bool sddmm, hipStream_t s) {
     return launch_stream_any(a, grid, comp, sddmm, s);

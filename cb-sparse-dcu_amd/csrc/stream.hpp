@@ -11,7 +11,6 @@ namespace cbs {
 namespace dev { struct bf16_t; }
 
 constexpr int kMaxStreamThreads = 64 * kMaxTileItems;   // one wave per tile item
-constexpr int kPanelThreads = 64 * kPanelWaves;         // one panel unit per workgroup
 // The batched kernel allows fewer waves per workgroup (so each may hold more
 // registers for several right-hand sides): plans of a batch have tile
 // waves <= 10.
@@ -46,12 +45,8 @@ struct StreamArgs {
     // column panels (panels > 1; cb_plan.hpp): n_items counts the tiles'
     // items only, the panel items follow them
     uint32_t panels;
-    const uint32_t* units;        // per unit: item begin, item end, panel, tile
-    const uint32_t* unit_ptr;     // panel q's units: [unit_ptr[q], unit_ptr[q + 1])
-    const uint32_t* tile_rows;    // per tile with panel rows: first row, last row
-    const uint32_t* tile_group;   // per tile: its tile_rows pair or ~0
-    const uint8_t* panel_mask;    // per row: panels holding entries of the row
-    A* panel_out;                 // panels x m partial sums
+    const uint32_t* panel_item_ptr;  // panel q's items: [ptr[q], ptr[q + 1])
+    A* panel_out;                    // panels x m partial sums
 };
 
 // One plan of a batched launch: its stream arguments (x, y, partial and
@@ -79,11 +74,12 @@ hipError_t launch_batch(const BatchPlan<float, float>* p, uint32_t np, uint32_t 
 hipError_t launch_batch(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t units, int grid, int threads,
                         size_t lds, bool comp, hipStream_t s);
 
-// Panel units of a plan with column panels (grid: a multiple of 8); they
-// must run before the tiles' launch on the same stream.
-hipError_t launch_panels(const StreamArgs<double, double>& a, int grid, bool sddmm, hipStream_t s);
-hipError_t launch_panels(const StreamArgs<float, float>& a, int grid, bool sddmm, hipStream_t s);
-hipError_t launch_panels(const StreamArgs<dev::bf16_t, float>& a, int grid, bool sddmm, hipStream_t s);
+// y[r] += the panel sums of row r, for every row with panel entries (after
+// a stream launch of a plan with column panels, on the same stream).
+hipError_t launch_panel_sums(double* y, const double* panel_out, const uint8_t* mask, uint32_t m, uint32_t panels,
+                             hipStream_t s);
+hipError_t launch_panel_sums(float* y, const float* panel_out, const uint8_t* mask, uint32_t m, uint32_t panels,
+                             hipStream_t s);
 
 // sddmm: also write dvals (a.xrow, a.dvals set)
 hipError_t launch_stream(const StreamArgs<double, double>& a, int grid, bool sddmm, hipStream_t s);

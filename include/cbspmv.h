@@ -132,9 +132,8 @@ int cbspmv_plan_stats(const cbspmv_plan* plan, cbspmv_stats* stats);
  * Stream schedule (u32 unless marked): stream_key stream_val(f64)
  *   stream_slab(u8) stream_items (4 per item) stream_tiles stream_tile_lines
  *   stream_empty_rows stream_value_pos(i64) stream_col_bits(i32)
- *   stream_panels (i32: panels, tile item count) stream_units (4 per unit)
- *   stream_unit_ptr stream_tile_rows (2 per tile with panel rows) stream_tile_group
- *   stream_panel_mask(u8)
+ *   stream_panels (i32: panels, tile item count) stream_panel_items (panel q's
+ *   items: [v[q], v[q + 1])) stream_panel_mask(u8)
  * Returns CBSPMV_ERR_ARG for an unknown name. */
 int cbspmv_export(const cbspmv_plan* plan, const char* name, void* dst, size_t* count);
 /* Byte offset of a slab region in the canonical image ("total" = size). */

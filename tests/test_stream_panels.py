@@ -1,9 +1,9 @@
 """CPU checks of the column-panel layout of the stream schedule: entries a
-tile cannot serve from LDS move to panel units; decoding tiles plus units
-gives back the input matrix, every unit's columns lie in its panel, its rows
-in its tile's row range, the per-row panel masks match the units, and every
-row with panel entries keeps an entry in the tiles (so the tile stores its y
-before it adds the panel sums)."""
+tile cannot serve from LDS move to per-panel item streams; decoding tiles
+plus panel items gives back the input matrix, every panel item's columns lie
+in its panel and no item cuts a row of its panel, the per-row panel masks
+match the items, and every row with panel entries keeps an entry in the
+tiles (so the tile stores its y before the panel sums are added)."""
 import numpy as np
 import pytest
 
@@ -45,36 +45,25 @@ def _check_panels(plan, m, n, r, c, v, dtype, expect_panels):
     if P == 1:
         assert n_tile_items == items.shape[0]
         return
-    units = plan.export("stream_units").reshape(-1, 4).astype(np.int64)
-    uptr = plan.export("stream_unit_ptr").astype(np.int64)
-    groups = plan.export("stream_tile_rows").reshape(-1, 2).astype(np.int64)
-    tgroup = plan.export("stream_tile_group").astype(np.int64)
+    ptr = plan.export("stream_panel_items").astype(np.int64)
     mask = plan.export("stream_panel_mask")
-    assert uptr.size == P + 1 and uptr[0] == 0 and uptr[-1] == units.shape[0]
-    assert tgroup.size == tiles.shape[0]
-    # units tile the panel items in order; panel q's units are contiguous
-    assert units[0, 0] == n_tile_items and units[-1, 1] == items.shape[0]
-    assert np.all(units[1:, 0] == units[:-1, 1]) and np.all(units[:, 1] > units[:, 0])
-    for q in range(P):
-        assert np.all(units[uptr[q]:uptr[q + 1], 2] == q)
+    assert ptr.size == P + 1 and ptr[0] == n_tile_items and ptr[-1] == items.shape[0]
+    assert np.all(np.diff(ptr) >= 0)
     width = -(-(-(-n // P)) // 16) * 16
     item_of = np.repeat(np.arange(items.shape[0]), items[:, 1] - items[:, 0])
     in_panel = item_of >= n_tile_items
     # panel items hold no staged keys (only columns and padding)
     cached = (plan.export("stream_key") & np.uint32(0x80000000)) != 0
     assert np.all(pad[in_panel] == cached[in_panel])
-    unit_of_item = np.full(items.shape[0], -1)
-    for u, (ib, ie, q, g) in enumerate(units):
-        unit_of_item[ib:ie] = u
-    uo = unit_of_item[item_of[in_panel & keep]]
-    prow, pcol = row[in_panel & keep], col[in_panel & keep]
-    assert np.all(pcol // width == units[uo, 2])
-    g = tgroup[units[uo, 3]]
-    assert np.all(g != 0xffffffff)
-    assert np.all((prow >= groups[g, 0]) & (prow <= groups[g, 1]))
-    # every tile with panel rows has units, and only those
-    assert sorted(tgroup[tgroup != 0xffffffff].tolist()) == list(range(groups.shape[0]))
-    assert set(np.nonzero(tgroup != 0xffffffff)[0].tolist()) == set(units[:, 3].tolist())
+    panel_of_item = np.searchsorted(ptr, np.arange(items.shape[0]), side="right") - 1
+    sel = in_panel & keep
+    prow, pcol = row[sel], col[sel]
+    assert np.all(pcol // width == panel_of_item[item_of[sel]])
+    # panel items never cut a row: each (row, panel) lies in one item
+    pair = prow * P + pcol // width
+    first_item = {}
+    for pr, it in zip(pair.tolist(), item_of[sel].tolist()):
+        assert first_item.setdefault(pr, it) == it
     # row masks
     want = np.zeros(m, np.uint8)
     np.bitwise_or.at(want, prow, (1 << (pcol // width)).astype(np.uint8))
@@ -83,19 +72,11 @@ def _check_panels(plan, m, n, r, c, v, dtype, expect_panels):
     tile_rows = np.zeros(m, bool)
     tile_rows[row[~in_panel]] = True
     assert np.all(tile_rows[mask != 0])
-    # a group's rows start at its tile's first row
-    tw = tile_waves(plan)
-    for t in np.nonzero(tgroup != 0xffffffff)[0]:
-        assert groups[tgroup[t], 0] == items[t * tw, 2]
     # item row offsets fit the key field
     cb = int(plan.export("stream_col_bits")[0])
     span = np.zeros(items.shape[0], np.int64)
     np.maximum.at(span, item_of, row - items[item_of, 2])
     assert np.all(span <= (1 << (31 - cb)) - 1)
-
-
-def tile_waves(plan):
-    return int(plan.export("stream_tile_shape")[0])
 
 
 @pytest.mark.parametrize("panels", [2, 4, 8])
@@ -135,7 +116,9 @@ def test_panels_keep_cut_rows_in_tiles():
     assert not np.any((row == 1500) & (item_of >= n_tile_items))
 
 
-def test_bad_panel_count_rejected():
+def test_bad_panel_options_rejected():
     m, n, r, c, v = _mixed(100, 100, 1, 5, seed=1)
     with pytest.raises(Exception, match="stream_panels"):
         Plan(m, n, r, c, v, panels=3)
+    with pytest.raises(Exception, match="stream_panels"):
+        Plan(m, n, r, c, v, panels=2, tile_waves=12)
