@@ -4,9 +4,10 @@
 // the tile's hot x cache lines into LDS with coalesced loads, so entries whose
 // column is staged read LDS instead of issuing a divergent gather to L2 (one
 // L2 request per lane), which is what bounds a gather-heavy SpMV here.  Then
-// each wave streams its item: key and value loads for chunk c+1 are issued
-// (non-temporal) before chunk c is reduced, so every wave keeps a chunk of
-// HBM reads in flight.  Rows are contiguous runs inside an item, so a chunk is
+// each wave streams its item in chunks of 64 lanes' entries, non-temporal:
+// with x in L2 one chunk ahead (stream_item_near), with x beyond L2 a ring of
+// chunks in flight through buffer instructions (stream_item), chosen per
+// launch by the size of x.  Rows are contiguous runs inside an item, so a chunk is
 // reduced with one DPP segmented scan, each finished row is stored to y by its
 // last lane, and the row still open at the end of a chunk is carried into
 // the next one.  Items are cut at row boundaries, so rows have one writer and
@@ -18,6 +19,10 @@
 // dx = A^T g it writes, for every stream entry (row j of A^T, column i),
 // dvals = g[i] * x[j] in stream order, reusing the g[i] the product gathers
 // anyway; lanes own consecutive entries, so those stores are coalesced.
+//
+// With column panels (cb_plan.hpp) every workgroup also runs, on as many
+// extra waves, items of its XCD's panel into per-panel row sums, and
+// panel_sums adds those to y in a second launch.
 #include <algorithm>
 #include <type_traits>
 
