@@ -121,3 +121,32 @@ def test_panels_match_whole_x(gpu):
     a = Plan(m, n, r, c, v, panels=1).spmv(x).cpu().numpy()
     b = Plan(m, n, r, c, v, panels=8).spmv(x).cpu().numpy()
     assert pyoracle.rel_l2(a, b) <= 1e-14
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("panels", [2, 8])
+def test_panels_small_cache_and_edges(gpu, panels):
+    """A 4 KB LDS cache (31 lines) leaves most entries to the panels; few
+    tiles (a grid of 8 workgroups); an empty matrix and one row."""
+    import torch
+
+    from cbspmv import gen
+
+    m, n = 3000, 20000
+    r, c, v = gen.random_coo(m, n, 7, seed=panels)
+    x = np.random.default_rng(9).uniform(-1, 1, n)
+    p = Plan(m, n, r, c, v, panels=panels, tile_kib=4)
+    assert int(p.export("stream_panels")[0]) == panels
+    y = p.spmv(_t(x, gpu)).cpu().numpy()
+    assert pyoracle.rel_l2(y, pyoracle.spmv_seq(m, r, c, v, x)) <= TOL
+    p.close()
+    e = Plan(40, 30, np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0), panels=panels)
+    y = e.spmv(_t(np.ones(30), gpu), _t(np.full(40, 7.0), gpu))
+    assert torch.count_nonzero(y).item() == 0
+    e.close()
+    oc = np.sort(np.random.default_rng(1).choice(n, 50, replace=False)).astype(np.int32)
+    ov = np.arange(50, dtype=np.float64)
+    one = Plan(1, n, np.zeros(50, np.int32), oc, ov, panels=panels, tile_kib=4)
+    y1 = one.spmv(_t(x, gpu)).cpu().numpy()
+    assert pyoracle.rel_l2(y1, pyoracle.spmv_seq(1, np.zeros(50, np.int32), oc, ov, x)) <= TOL
+    one.close()

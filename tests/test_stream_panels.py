@@ -122,3 +122,13 @@ def test_bad_panel_options_rejected():
         Plan(m, n, r, c, v, panels=3)
     with pytest.raises(Exception, match="stream_panels"):
         Plan(m, n, r, c, v, panels=2, tile_waves=12)
+
+
+@pytest.mark.parametrize("panels", [2, 8])
+def test_small_cache_panels_layout(panels):
+    # a 4 KB LDS cache leaves most entries to the panels
+    r, c, v = gen.random_coo(3000, 20000, 7, seed=panels)
+    plan = Plan(3000, 20000, r, c, v, panels=panels, tile_kib=4)
+    _check_panels(plan, 3000, 20000, r, c, v, "f64", panels)
+    n_tile_items = int(plan.export("stream_panels")[1])
+    assert plan.export("stream_items").reshape(-1, 4).shape[0] > n_tile_items
