@@ -10,7 +10,8 @@ from cbspmv import CbspmvError, Plan, gen
 from oracle import pyoracle
 
 NAMES = pyoracle.ARRAY_NAMES + ["image", "stream_slab", "stream_key", "stream_val", "stream_items", "stream_tiles",
-                                "stream_tile_lines", "stream_empty_rows", "policy", "stream_col_bits"]
+                                "stream_tile_lines", "stream_empty_rows", "policy", "stream_col_bits", "stream_panels",
+                                "stream_panel_items", "stream_panel_mask"]
 
 
 def _plans():
@@ -20,6 +21,8 @@ def _plans():
     yield "dup_lpt_bs32", (900, 700, r, c, v), dict(block_size=32, row_owned=False)
     yield "f32", (900, 700, r, c, v), dict(dtype="f32", compensated=True)
     yield "bf16", (900, 700, r, c, v), dict(dtype="bf16")
+    r2, c2, v2 = gen.random_coo(3000, 20000, 7, seed=5)
+    yield "panels", (3000, 20000, r2, c2, v2), dict(panels=4, tile_kib=4)
 
 
 @pytest.mark.parametrize("name,mat,opts", list(_plans()), ids=lambda v: v if isinstance(v, str) else "")
