@@ -75,7 +75,11 @@ int cbspmv_build_transpose(cbspmv_plan* plan) {
             tval[q] = h.in_val[perm[q]];
         }
         cbspmv_plan* t = nullptr;
-        int rc = cbs::build_plan_object(plan->opts, n, m, nnz, trow.data(), tcol.data(), tval.data(), false, &t);
+        // the backward launch (with the SDDMM stores) is fastest at 8 waves
+        // per tile for every value type
+        cbspmv_opts to = plan->opts;
+        if (!to.stream_tile_waves) to.stream_tile_waves = cbs::kDefaultTileItemsOpt;
+        int rc = cbs::build_plan_object(to, n, m, nnz, trow.data(), tcol.data(), tval.data(), false, &t);
         if (rc) return rc;
         std::vector<uint32_t> own;
         if (!cbs::stream_entry_slots(t->host, own) || !cbs::stream_entry_slots(plan->host, plan->slot_of)) {

@@ -169,10 +169,6 @@ int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** bat
             delete b;
             return fail(CBSPMV_ERR_ARG, "plans of one batch must share value type and accumulation");
         }
-        if (h.o.tile_items > cbs::kMaxBatchTileItems) {
-            delete b;
-            return fail(CBSPMV_ERR_UNSUPPORTED, "plans of a batch need stream tile waves <= 10");
-        }
         if (h.panels > 1) {
             delete b;
             return fail(CBSPMV_ERR_UNSUPPORTED, "plans of a batch need stream_panels = 1");
@@ -187,7 +183,8 @@ int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** bat
         b->units += std::max<uint32_t>(1u, (uint32_t)(h.tiles.size() / 2));
         partial_elems += (size_t)h.partial_slots * cbs::kBatchMaxRhs * 2;   // room for (hi, lo)
         counter_elems += h.splits.size() * cbs::kBatchMaxRhs;
-        max_items = std::max(max_items, h.o.tile_items);
+        // waves per workgroup: a tile with more items than waves deals them round-robin
+        max_items = std::max(max_items, std::min(h.o.tile_items, cbs::kMaxBatchTileItems));
         max_slots = std::max(max_slots, cbs::tile_slots(h.o.tile_bytes, h.o.dtype));
     }
     const size_t eb = b->dtype == CBSPMV_F64 ? 8 : 4;

@@ -57,9 +57,10 @@ int build_plan_object(const cbspmv_opts& o, int m, int n, int nnz, const int* ro
     h.o.accum = o.accumulate;
     h.o.schedule = canonical ? o.schedule : CBSPMV_SCHED_STREAM;
     h.o.item_nnz = o.stream_item_nnz;
-    if (o.stream_tile_waves) h.o.tile_items = o.stream_tile_waves;
-    if (o.stream_tile_kib) h.o.tile_bytes = o.stream_tile_kib * 1024;
     h.o.panels = o.stream_panels;
+    if (o.stream_tile_waves) h.o.tile_items = o.stream_tile_waves;
+    else if (o.value_dtype != CBSPMV_F64 && h.o.panels <= 1) h.o.tile_items = kWideTileItemsOpt;
+    if (o.stream_tile_kib) h.o.tile_bytes = o.stream_tile_kib * 1024;
     if (h.o.panels > 1 && h.o.tile_items > kMaxTileItems / 2) {
         delete p;
         return fail(CBSPMV_ERR_ARG, "stream_panels > 1 needs stream_tile_waves <= 8 (a workgroup adds as many panel waves)");

@@ -590,14 +590,24 @@ __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t 
 }
 
 // One tile of the stream: stage its hot x lines in LDS, then each of its
-// first tile_items waves streams one item.  Every thread of the workgroup
-// calls this.
-template <typename V, typename A, bool COMP, int EPL, bool SDDMM, int RING>
+// first tile_items waves streams one item.  With LOOP the workgroup may have
+// fewer waves than the tile has items: wave w streams items w, w + waves, ...
+// Every thread of the workgroup calls this.
+template <typename V, typename A, bool COMP, int EPL, bool SDDMM, int RING, bool LOOP = false>
 __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t tile, A* xs) {
     stage_tile(a, tile, xs);
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // wave-uniform
-    const uint32_t item = tile * a.tile_items + wave;
-    if (wave < a.tile_items && item < a.n_items) stream_item<V, A, COMP, EPL, SDDMM, true, RING>(a, item, xs, a.y);
+    if constexpr (LOOP) {
+        const uint32_t waves = blockDim.x >> 6;
+        for (uint32_t w = wave; w < a.tile_items; w += waves) {
+            const uint32_t item = tile * a.tile_items + w;
+            if (item >= a.n_items) break;
+            stream_item<V, A, COMP, EPL, SDDMM, true, RING>(a, item, xs, a.y);
+        }
+    } else {
+        const uint32_t item = tile * a.tile_items + wave;
+        if (wave < a.tile_items && item < a.n_items) stream_item<V, A, COMP, EPL, SDDMM, true, RING>(a, item, xs, a.y);
+    }
 }
 
 // Several right-hand sides at once: each chunk of keys and values is loaded
@@ -606,22 +616,15 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t 
 // from global memory (L2) for every entry; LDS holds the tile's line table
 // to decode the column of an entry whose key names an LDS slot.  RHS r
 // reads x + r*ldx and writes y + r*ldy; pieces of cut rows go to
-// partial + r*pstride and share one ticket per split.
+// partial + r*pstride and share one ticket per split.  multi_item runs one
+// item with one wave; stream_tile_multi deals the tile's items to the
+// workgroup's waves.
 template <typename V, typename A, bool COMP, int EPL, int R>
-__device__ __forceinline__ void stream_tile_multi(const StreamArgs<V, A>& a, uint32_t tile, uint32_t* lines,
-                                                  uint64_t ldx, uint64_t ldy, uint64_t pstride, uint32_t nr) {
+__device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t item, const uint32_t* lines,
+                                           uint64_t ldx, uint64_t ldy, uint64_t pstride, uint32_t nr) {
     using Acc = typename std::conditional<COMP, f2, A>::type;
     const uint32_t kZero = a.tile_slots - 1;
     const int lane = lane_id();
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // wave-uniform
-    {
-        const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1];
-        for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) lines[s] = a.tile_lines[lb + s];
-    }
-    __syncthreads();
-    if ((uint32_t)wave >= a.tile_items) return;
-    const uint32_t item = tile * a.tile_items + wave;
-    if (item >= a.n_items) return;
     const StreamItem it = a.items[item];
     const int cb = a.col_bits;
     const uint32_t cmask = (1u << cb) - 1u;
@@ -757,6 +760,23 @@ __device__ __forceinline__ void stream_tile_multi(const StreamArgs<V, A>& a, uin
     }
 }
 
+template <typename V, typename A, bool COMP, int EPL, int R>
+__device__ __forceinline__ void stream_tile_multi(const StreamArgs<V, A>& a, uint32_t tile, uint32_t* lines,
+                                                  uint64_t ldx, uint64_t ldy, uint64_t pstride, uint32_t nr) {
+    {
+        const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1];
+        for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) lines[s] = a.tile_lines[lb + s];
+    }
+    __syncthreads();
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // wave-uniform
+    const uint32_t waves = blockDim.x >> 6;
+    for (uint32_t w = wave; w < a.tile_items; w += waves) {
+        const uint32_t item = tile * a.tile_items + w;
+        if (item >= a.n_items) break;
+        multi_item<V, A, COMP, EPL, R>(a, item, lines, ldx, ldy, pstride, nr);
+    }
+}
+
 template <typename V, typename A, bool COMP, int EPL, bool SDDMM, bool NEAR>
 __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -836,7 +856,7 @@ __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan
             a.y = bp.y;
             a.partial = bp.partial;
             a.counters = bp.counters;
-            stream_tile<V, A, COMP, EPL, false, kNearRing>(a, tile, xs);
+            stream_tile<V, A, COMP, EPL, false, kNearRing, true>(a, tile, xs);
             __syncthreads();   // LDS is restaged by the next unit
             continue;
         }

@@ -11,9 +11,9 @@ namespace cbs {
 namespace dev { struct bf16_t; }
 
 constexpr int kMaxStreamThreads = 64 * kMaxTileItems;   // one wave per tile item
-// The batched kernel allows fewer waves per workgroup (so each may hold more
-// registers for several right-hand sides): plans of a batch have tile
-// waves <= 10.
+// The batched kernel has at most 10 waves per workgroup (so each may hold
+// more registers for several right-hand sides); the items of a tile with
+// more are dealt to its waves in turn.
 constexpr int kMaxBatchTileItems = 10;
 constexpr int kMaxBatchThreads = 64 * kMaxBatchTileItems;
 

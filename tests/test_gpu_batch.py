@@ -70,10 +70,14 @@ def test_batch_parity(gpu, dtype, comp):
     tdt = torch.float64 if dtype == "f64" else torch.float32
     rng = np.random.default_rng(1)
     plans, X, Y = [], [], []
+    # tiles of more than 10 items (12, 16) are dealt to the batch kernel's
+    # 10 waves in turn, on both the one- and the several-right-hand-side path
+    waves = [4, 9, 16, 12, 8, 16, 4, 12, 9, 12, 16]
+    rhs = [1, 3, 1, 16, 8, 8, 16, 1, 3, 16, 1]
     for i, (m, n, r, c, v) in enumerate(mats):
-        opts = dict(dtype=dtype, compensated=comp, tile_waves=[4, 9, 8][i % 3], item_nnz=[0, 256, 0][i % 3])
+        opts = dict(dtype=dtype, compensated=comp, tile_waves=waves[i % 11], item_nnz=[0, 256, 0][i % 3])
         plans.append(Plan(m, n, r, c, v, **opts))
-        k = [1, 3, 8, 16][i % 4]
+        k = rhs[i % 11]
         X.append(torch.tensor(rng.uniform(-1, 1, (k, n)), dtype=tdt, device=gpu))
         Y.append(torch.full((k, m), 7.0, dtype=tdt, device=gpu))
     print(f"batch {dtype}: {len(plans)} plans", flush=True)
