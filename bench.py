@@ -106,12 +106,17 @@ def cpu_baseline(d, budget_s):
 
 
 VENDOR_ALGS = {"csr_adaptive": 2, "csr_rowsplit": 3, "csr_lrb": 7, "csr_nnzsplit": 8}
+BSR_BLOCK_DIMS = (2, 4, 8, 16)   # the reference's hipSPARSE BSR baseline sizes (benchmark/hipSparse-bsr/macros.h:9)
 
 
 def vendor_baseline(d, dtype, reps, ours_ms):
-    """rocSPARSE CSR SpMV (libvendor_csr.so) on the same matrix and GPU, each
-    algorithm timed like ours; the comparator the reference's hipSPARSE
-    baseline programs stand for."""
+    """rocSPARSE CSR SpMV (each rocsparse_spmv algorithm) and BSR SpMV
+    (rocsparse_bsrmv at block dims 2, 4, 8, 16 after rocsparse_csr2bsr) from
+    libvendor_csr.so on the same matrix and GPU, timed like ours; the
+    comparators the reference's hipSPARSE baseline programs stand for
+    (benchmark/hipSparse-csr/main.hip.cpp:104-181,
+    benchmark/hipSparse-bsr/main.hip.cpp:99-187).  BSR launches move whole
+    blocks (up to 256 values each), so they run fewer timed launches."""
     import ctypes
 
     import numpy as np
@@ -125,11 +130,15 @@ def vendor_baseline(d, dtype, reps, ours_ms):
     ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
     L.vendor_csr_spmv.restype = ctypes.c_int
     L.vendor_csr_spmv.argtypes = [ctypes.c_int] * 3 + [ip, ip, dp, dp, dp] + [ctypes.c_int] * 4 + [dp]
+    L.vendor_bsr_spmv.restype = ctypes.c_int
+    L.vendor_bsr_spmv.argtypes = ([ctypes.c_int] * 3 + [ip, ip, dp, dp, dp] + [ctypes.c_int] * 4 +
+                                  [dp, ctypes.POINTER(ctypes.c_longlong)])
     m, n, r, c, v, x = d["m"], d["n"], d["row"], d["col"], d["val"], d["x"]
     ptr = np.zeros(m + 1, np.int32)
     np.cumsum(np.bincount(r, minlength=m), out=ptr[1:])
-    col = np.ascontiguousarray(c, np.int32)
-    val = np.ascontiguousarray(v, np.float64)
+    order = np.lexsort((c, r))   # rocSPARSE's CSR: columns ascending within a row
+    col = np.ascontiguousarray(c[order], np.int32)
+    val = np.ascontiguousarray(v[order], np.float64)
     xx = np.ascontiguousarray(x, np.float64)
     y = np.zeros(m)
     y_ref = pyoracle.spmv_csr_omp(m, r, c, v, x, min(16, os.cpu_count() or 1))
@@ -146,9 +155,23 @@ def vendor_baseline(d, dtype, reps, ours_ms):
         out[name] = {"us": round(ms.value * 1e3, 2), "GB_s": round(b_alg / (ms.value * 1e-3) / 1e9, 1),
                      "rel_l2_vs_fp64": pyoracle.rel_l2(y, y_ref)}
         log(f"rocSPARSE {name}: {ms.value * 1e3:.1f} us")
+    for bd in BSR_BLOCK_DIMS:
+        name = f"bsr_{bd}"
+        ms, nnzb = ctypes.c_double(), ctypes.c_longlong()
+        y[:] = 0.0
+        rc = L.vendor_bsr_spmv(m, n, int(r.size), ptr.ctypes.data_as(ip), col.ctypes.data_as(ip),
+                               val.ctypes.data_as(dp), xx.ctypes.data_as(dp), y.ctypes.data_as(dp), bd,
+                               int(dtype != "f64"), 3, max(5, reps // 10), ctypes.byref(ms), ctypes.byref(nnzb))
+        if rc != 0:
+            out[name] = {"status": rc}
+            continue
+        out[name] = {"us": round(ms.value * 1e3, 2), "GB_s": round(b_alg / (ms.value * 1e-3) / 1e9, 1),
+                     "blocks": int(nnzb.value), "rel_l2_vs_fp64": pyoracle.rel_l2(y, y_ref)}
+        log(f"rocSPARSE {name}: {ms.value * 1e3:.1f} us ({nnzb.value} blocks)")
     ok = {k: e for k, e in out.items() if "us" in e}
     best = min(ok, key=lambda k: ok[k]["us"]) if ok else None
-    return {"library": "rocSPARSE (ROCm) CSR rocsparse_spmv, same matrix and GPU", "algorithms": out,
+    return {"library": "rocSPARSE (ROCm): CSR rocsparse_spmv and BSR rocsparse_bsrmv, same matrix and GPU",
+            "algorithms": out,
             "best": best, "best_us": ok[best]["us"] if best else None,
             "cbspmv_speedup_vs_best": round(ok[best]["us"] / (ours_ms * 1e3), 3) if best else None}
 

@@ -168,3 +168,12 @@ def test_report_helpers(tmp_path):
     buf = io.StringIO()
     print_benchmark_report("tiny", 3, 4, 4, gb, met, True, file=buf)
     assert "CB-SpMV Benchmark" in buf.getvalue() and "PASS" in buf.getvalue()
+
+
+def test_vendor_comparator_library_exports():
+    """bench.py's rocSPARSE comparators (CSR and BSR) are built by build()
+    beside the library and export both entry points."""
+    path = os.path.join(ROOT, "cb-sparse-dcu_amd", "libvendor_csr.so")
+    if not os.path.exists(path):
+        pytest.skip("libvendor_csr.so not built")
+    assert {"vendor_csr_spmv", "vendor_bsr_spmv"} <= exported_symbols(path)
