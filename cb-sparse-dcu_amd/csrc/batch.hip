@@ -37,8 +37,13 @@ struct cbspmv_batch {
     int grid = 8;
     cbs::DeviceBuf desc, partial, counters;
     std::vector<uint8_t> host_desc;   // last uploaded descriptor bytes
-    // captured graph and the descriptor it reads
-    cbs::DeviceBuf graph_desc;
+    // x interleaved per group of right-hand sides (plans with several) and
+    // the largest groups * n, for eager calls
+    cbs::DeviceBuf xg;
+    uint32_t xg_cols = 0;
+    // captured graph, the descriptor and interleave buffer it reads
+    cbs::DeviceBuf graph_desc, graph_xg;
+    uint32_t graph_xg_cols = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -69,19 +74,39 @@ void fill_args(const cbspmv_plan& p, StreamArgs<V, A>& a) {
     a.n = h.n;
 }
 
-// Descriptor bytes for one call (x/y pointer lists and RHS counts).
+constexpr uint32_t kRhsGroup = 4;   // right-hand sides per interleaved group (stream.hip kMultiRhs)
+
+// Descriptor bytes for one call (x/y pointer lists and RHS counts).  Plans
+// with several right-hand sides get a slice of xg (grown here when too
+// small) for their interleaved x; max_cols = the largest groups * n.
 template <typename V, typename A>
-int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* nrhs, std::vector<uint8_t>& out) {
+int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* nrhs, DeviceBuf& xg,
+              uint32_t& max_cols, std::vector<uint8_t>& out, hipStream_t s) {
     const size_t np = b.plans.size();
+    size_t xg_elems = 0;
+    max_cols = 0;
+    for (size_t i = 0; i < np; ++i) {
+        const Plan& h = b.plans[i]->host;
+        if (nrhs[i] < 0 || nrhs[i] > kBatchMaxRhs) return fail(CBSPMV_ERR_ARG, "nrhs must be in 0..16");
+        if (nrhs[i] > 0 && ((h.m > 0 && !y[i]) || (h.nnz > 0 && !x[i]))) return fail(CBSPMV_ERR_ARG, "null vector");
+        if (nrhs[i] > 1) {
+            const uint32_t cols = ((uint32_t)nrhs[i] + kRhsGroup - 1) / kRhsGroup * (uint32_t)h.n;
+            xg_elems += (size_t)cols * kRhsGroup;
+            max_cols = std::max(max_cols, cols);
+        }
+    }
+    if (xg.bytes < xg_elems * sizeof(A)) {
+        const int rc = upload_bytes(xg, nullptr, xg_elems * sizeof(A), s);
+        if (rc) return rc;
+    }
     std::vector<BatchPlan<V, A>> d(np);
     A* part = static_cast<A*>(b.partial.ptr);
     uint32_t* cnt = static_cast<uint32_t*>(b.counters.ptr);
+    A* xgp = static_cast<A*>(xg.ptr);
     uint32_t unit = 0;
     for (size_t i = 0; i < np; ++i) {
         const cbspmv_plan& p = *b.plans[i];
         const Plan& h = p.host;
-        if (nrhs[i] < 0 || nrhs[i] > kBatchMaxRhs) return fail(CBSPMV_ERR_ARG, "nrhs must be in 0..16");
-        if (nrhs[i] > 0 && ((h.m > 0 && !y[i]) || (h.nnz > 0 && !x[i]))) return fail(CBSPMV_ERR_ARG, "null vector");
         BatchPlan<V, A>& e = d[i];
         fill_args(p, e.args);
         e.unit_begin = unit;
@@ -96,6 +121,11 @@ int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* 
         e.counter_stride = h.splits.size();
         e.partial = part;
         e.counters = cnt;
+        e.xg = nullptr;
+        if (nrhs[i] > 1) {
+            e.xg = xgp;
+            xgp += (size_t)((uint32_t)nrhs[i] + kRhsGroup - 1) / kRhsGroup * kRhsGroup * (size_t)h.n;
+        }
         part += (size_t)h.partial_slots * parts * kBatchMaxRhs;
         cnt += h.splits.size() * kBatchMaxRhs;
     }
@@ -104,10 +134,11 @@ int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* 
     return CBSPMV_OK;
 }
 
-int build_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* nrhs, std::vector<uint8_t>& out) {
-    if (b.dtype == CBSPMV_F64) return make_desc<double, double>(b, x, y, nrhs, out);
-    if (b.dtype == CBSPMV_F32) return make_desc<float, float>(b, x, y, nrhs, out);
-    return make_desc<dev::bf16_t, float>(b, x, y, nrhs, out);
+int build_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* nrhs, DeviceBuf& xg,
+               uint32_t& max_cols, std::vector<uint8_t>& out, hipStream_t s) {
+    if (b.dtype == CBSPMV_F64) return make_desc<double, double>(b, x, y, nrhs, xg, max_cols, out, s);
+    if (b.dtype == CBSPMV_F32) return make_desc<float, float>(b, x, y, nrhs, xg, max_cols, out, s);
+    return make_desc<dev::bf16_t, float>(b, x, y, nrhs, xg, max_cols, out, s);
 }
 
 int upload_desc(DeviceBuf& buf, const std::vector<uint8_t>& bytes, hipStream_t s) {
@@ -122,19 +153,25 @@ int upload_desc(DeviceBuf& buf, const std::vector<uint8_t>& bytes, hipStream_t s
     return CBSPMV_OK;
 }
 
-int launch(cbspmv_batch& b, const void* desc, hipStream_t s) {
+// The interleave launch (plans with several right-hand sides), then the
+// batched launch, on stream s.
+int launch(cbspmv_batch& b, const void* desc, uint32_t xg_cols, hipStream_t s) {
     if (b.plans.empty()) return CBSPMV_OK;
     hipError_t e;
     const uint32_t np = (uint32_t)b.plans.size();
-    if (b.dtype == CBSPMV_F64)
-        e = launch_batch(static_cast<const BatchPlan<double, double>*>(desc), np, b.units, b.grid, b.threads, b.lds,
-                         false, s);
-    else if (b.dtype == CBSPMV_F32)
-        e = launch_batch(static_cast<const BatchPlan<float, float>*>(desc), np, b.units, b.grid, b.threads, b.lds,
-                         b.comp, s);
-    else
-        e = launch_batch(static_cast<const BatchPlan<dev::bf16_t, float>*>(desc), np, b.units, b.grid, b.threads,
-                         b.lds, b.comp, s);
+    if (b.dtype == CBSPMV_F64) {
+        const auto* d = static_cast<const BatchPlan<double, double>*>(desc);
+        e = launch_interleave(d, np, xg_cols, s);
+        if (e == hipSuccess) e = launch_batch(d, np, b.units, b.grid, b.threads, b.lds, false, s);
+    } else if (b.dtype == CBSPMV_F32) {
+        const auto* d = static_cast<const BatchPlan<float, float>*>(desc);
+        e = launch_interleave(d, np, xg_cols, s);
+        if (e == hipSuccess) e = launch_batch(d, np, b.units, b.grid, b.threads, b.lds, b.comp, s);
+    } else {
+        const auto* d = static_cast<const BatchPlan<dev::bf16_t, float>*>(desc);
+        e = launch_interleave(d, np, xg_cols, s);
+        if (e == hipSuccess) e = launch_batch(d, np, b.units, b.grid, b.threads, b.lds, b.comp, s);
+    }
     return e == hipSuccess ? CBSPMV_OK : hip_fail(e, "batched launch");
 }
 
@@ -219,13 +256,13 @@ int cbspmv_batch_spmv(cbspmv_batch* batch, const void* const* d_x, void* const* 
     if (!d_x || !d_y || !nrhs) return fail(CBSPMV_ERR_ARG, "null argument");
     const hipStream_t s = static_cast<hipStream_t>(stream);
     std::vector<uint8_t> bytes;
-    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, bytes);
+    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, batch->xg, batch->xg_cols, bytes, s);
     if (rc) return rc;
     if (bytes != batch->host_desc) {
         if ((rc = cbs::upload_desc(batch->desc, bytes, s))) return rc;
         batch->host_desc.swap(bytes);
     }
-    return cbs::launch(*batch, batch->desc.ptr, s);
+    return cbs::launch(*batch, batch->desc.ptr, batch->xg_cols, s);
 }
 
 int cbspmv_batch_capture(cbspmv_batch* batch, const void* const* d_x, void* const* d_y, const int* nrhs,
@@ -234,13 +271,13 @@ int cbspmv_batch_capture(cbspmv_batch* batch, const void* const* d_x, void* cons
     if (!stream) return fail(CBSPMV_ERR_ARG, "graph capture needs a created (non-null) stream");
     if (!d_x || !d_y || !nrhs) return fail(CBSPMV_ERR_ARG, "null argument");
     const hipStream_t s = static_cast<hipStream_t>(stream);
+    cbs::drop_graph(*batch);   // before graph_xg may be reallocated
     std::vector<uint8_t> bytes;
-    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, bytes);
+    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, batch->graph_xg, batch->graph_xg_cols, bytes, s);
     if (rc) return rc;
     if ((rc = cbs::upload_desc(batch->graph_desc, bytes, s))) return rc;
-    cbs::drop_graph(*batch);
     CBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    rc = cbs::launch(*batch, batch->graph_desc.ptr, s);
+    rc = cbs::launch(*batch, batch->graph_desc.ptr, batch->graph_xg_cols, s);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -261,7 +298,9 @@ int cbspmv_batch_replay(cbspmv_batch* batch, void* stream) {
 void cbspmv_batch_destroy(cbspmv_batch* batch) {
     if (!batch) return;
     cbs::drop_graph(*batch);
-    for (cbs::DeviceBuf* d : {&batch->desc, &batch->partial, &batch->counters, &batch->graph_desc}) cbs::release(*d);
+    for (cbs::DeviceBuf* d : {&batch->desc, &batch->partial, &batch->counters, &batch->graph_desc, &batch->xg,
+                              &batch->graph_xg})
+        cbs::release(*d);
     delete batch;
 }
 

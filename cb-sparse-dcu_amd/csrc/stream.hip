@@ -610,18 +610,42 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t 
     }
 }
 
+// The R values x[col] of R right-hand sides, from x interleaved by column
+// (p = xg + col * R; 16-byte aligned): one 16-byte load per 4 floats or 2
+// doubles, so the R right-hand sides cost R/4 (FP32) or R/2 (FP64) gathers
+// per entry instead of R.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <int R>
+__device__ __forceinline__ void load_rhs(const float* p, float (&v)[R]) {
+#pragma unroll
+    for (int q = 0; q < R / 4; ++q) {
+        const f32x4 w = *reinterpret_cast<const f32x4*>(p + 4 * q);
+        v[4 * q] = w.x; v[4 * q + 1] = w.y; v[4 * q + 2] = w.z; v[4 * q + 3] = w.w;
+    }
+}
+template <int R>
+__device__ __forceinline__ void load_rhs(const double* p, double (&v)[R]) {
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) {
+        const f64x2 w = *reinterpret_cast<const f64x2*>(p + 2 * q);
+        v[2 * q] = w.x; v[2 * q + 1] = w.y;
+    }
+}
+
 // Several right-hand sides at once: each chunk of keys and values is loaded
 // once and reduced for every right-hand side r < nr (at most R), so the slab
-// is read once per tile instead of once per right-hand side.  x is gathered
-// from global memory (L2) for every entry; LDS holds the tile's line table
-// to decode the column of an entry whose key names an LDS slot.  RHS r
-// reads x + r*ldx and writes y + r*ldy; pieces of cut rows go to
-// partial + r*pstride and share one ticket per split.  multi_item runs one
-// item with one wave; stream_tile_multi deals the tile's items to the
-// workgroup's waves.
+// is read once per tile instead of once per right-hand side.  x comes
+// interleaved (a.x[col * R + r], written by interleave_rhs) and is gathered
+// from global memory (L2) for every entry, all R values of a column with
+// one vector load; LDS holds the tile's line table to decode the column of
+// an entry whose key names an LDS slot.  RHS r writes y + r*ldy; pieces of
+// cut rows go to partial + r*pstride and share one ticket per split.
+// multi_item runs one item with one wave; stream_tile_multi deals the tile's
+// items to the workgroup's waves.
 template <typename V, typename A, bool COMP, int EPL, int R>
 __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t item, const uint32_t* lines,
-                                           uint64_t ldx, uint64_t ldy, uint64_t pstride, uint32_t nr) {
+                                           uint64_t ldy, uint64_t pstride, uint32_t nr) {
     using Acc = typename std::conditional<COMP, f2, A>::type;
     const uint32_t kZero = a.tile_slots - 1;
     const int lane = lane_id();
@@ -692,14 +716,16 @@ __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t i
         const int prev_off = wave_shift<kWaveShr1>(off[EPL - 1], -1);
         const int next_off = wave_shift<kWaveShl1>(off[0], -1);
         const bool carried = carry_off >= 0 && o0 == carry_off;
+        A xr[EPL][R];
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) load_rhs<R>(a.x + (size_t)col[j] * R, xr[j]);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if ((uint32_t)r >= nr) break;
-            const A* x = a.x + (size_t)r * ldx;
             Acc s[EPL];
 #pragma unroll
             for (int j = 0; j < EPL; ++j) {
-                const A xv = x[col[j]];
+                const A xv = xr[j][r];
                 if constexpr (COMP) s[j] = two_prod(widen(cur.v[j]), xv);
                 else s[j] = A(widen(cur.v[j])) * xv;
             }
@@ -762,7 +788,7 @@ __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t i
 
 template <typename V, typename A, bool COMP, int EPL, int R>
 __device__ __forceinline__ void stream_tile_multi(const StreamArgs<V, A>& a, uint32_t tile, uint32_t* lines,
-                                                  uint64_t ldx, uint64_t ldy, uint64_t pstride, uint32_t nr) {
+                                                  uint64_t ldy, uint64_t pstride, uint32_t nr) {
     {
         const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1];
         for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) lines[s] = a.tile_lines[lb + s];
@@ -773,7 +799,7 @@ __device__ __forceinline__ void stream_tile_multi(const StreamArgs<V, A>& a, uin
     for (uint32_t w = wave; w < a.tile_items; w += waves) {
         const uint32_t item = tile * a.tile_items + w;
         if (item >= a.n_items) break;
-        multi_item<V, A, COMP, EPL, R>(a, item, lines, ldx, ldy, pstride, nr);
+        multi_item<V, A, COMP, EPL, R>(a, item, lines, ldy, pstride, nr);
     }
 }
 
@@ -860,20 +886,54 @@ __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan
             __syncthreads();   // LDS is restaged by the next unit
             continue;
         }
-        // several: groups of up to kMultiRhs share each chunk load
+        // several: groups of up to kMultiRhs share each chunk load and each
+        // x gather (x interleaved per group by interleave_rhs)
         for (uint32_t r0 = 0; r0 < bp.nrhs; r0 += kMultiRhs) {
             StreamArgs<V, A> a = bp.args;
-            a.x = bp.x + (size_t)r0 * bp.ldx;
+            a.x = bp.xg + (size_t)r0 * bp.args.n;
             a.y = bp.y + (size_t)r0 * bp.ldy;
             a.partial = bp.partial + (size_t)r0 * bp.partial_stride;
             a.counters = bp.counters + (size_t)r0 * bp.counter_stride;
-            stream_tile_multi<V, A, COMP, EPL, kMultiRhs>(a, tile, reinterpret_cast<uint32_t*>(xs), bp.ldx, bp.ldy,
+            stream_tile_multi<V, A, COMP, EPL, kMultiRhs>(a, tile, reinterpret_cast<uint32_t*>(xs), bp.ldy,
                                                           bp.partial_stride, min(bp.nrhs - r0, (uint32_t)kMultiRhs));
             __syncthreads();
         }
     }
 }
 
+// x of every plan with several right-hand sides, interleaved in groups of
+// kMultiRhs for stream_tile_multi: xg[(g*n + col)*kMultiRhs + j] =
+// x[(g*kMultiRhs + j)*ldx + col], 0 past the last right-hand side.  Reads
+// are coalesced over col, each thread writes one 16-byte (FP32) or two
+// 16-byte (FP64) vectors.
+__device__ __forceinline__ void store_rhs(float* p, const float (&v)[4]) {
+    *reinterpret_cast<f32x4*>(p) = f32x4{v[0], v[1], v[2], v[3]};
+}
+__device__ __forceinline__ void store_rhs(double* p, const double (&v)[4]) {
+    reinterpret_cast<f64x2*>(p)[0] = f64x2{v[0], v[1]};
+    reinterpret_cast<f64x2*>(p)[1] = f64x2{v[2], v[3]};
+}
+static_assert(kMultiRhs == 4, "store_rhs writes groups of 4");
+
+template <typename V, typename A>
+__global__ __launch_bounds__(256) void interleave_rhs(const BatchPlan<V, A>* plans, uint32_t nplans) {
+    for (uint32_t p = blockIdx.y; p < nplans; p += gridDim.y) {
+        const BatchPlan<V, A>& bp = plans[p];
+        if (bp.nrhs < 2) continue;
+        const uint32_t n = (uint32_t)bp.args.n;
+        const uint32_t total = (bp.nrhs + kMultiRhs - 1) / kMultiRhs * n;
+        for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+            const uint32_t g = t / n, col = t - g * n;
+            A v[kMultiRhs];
+#pragma unroll
+            for (int j = 0; j < kMultiRhs; ++j) {
+                const uint32_t r = g * kMultiRhs + (uint32_t)j;
+                v[j] = r < bp.nrhs ? bp.x[(size_t)r * bp.ldx + col] : A(0);
+            }
+            store_rhs(bp.xg + (size_t)t * kMultiRhs, v);
+        }
+    }
+}
 
 }  // namespace dev
 
@@ -933,6 +993,23 @@ int batch_blocks_per_cu(int dtype, bool comp, int threads, size_t lds) {
         e = comp ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::stream_batch<dev::bf16_t, float, true, 8>, threads, lds)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, dev::stream_batch<dev::bf16_t, float, false, 8>, threads, lds);
     return e == hipSuccess ? n : 0;
+}
+
+template <typename V, typename A>
+hipError_t launch_interleave_t(const BatchPlan<V, A>* p, uint32_t np, uint32_t max_cols, hipStream_t s) {
+    if (!np || !max_cols) return hipSuccess;
+    const dim3 grid(std::min<uint32_t>(32u, (max_cols + 255u) / 256u), std::min<uint32_t>(np, 65535u));
+    hipLaunchKernelGGL((dev::interleave_rhs<V, A>), grid, dim3(256), 0, s, p, np);
+    return hipGetLastError();
+}
+hipError_t launch_interleave(const BatchPlan<double, double>* p, uint32_t np, uint32_t max_cols, hipStream_t s) {
+    return launch_interleave_t(p, np, max_cols, s);
+}
+hipError_t launch_interleave(const BatchPlan<float, float>* p, uint32_t np, uint32_t max_cols, hipStream_t s) {
+    return launch_interleave_t(p, np, max_cols, s);
+}
+hipError_t launch_interleave(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t max_cols, hipStream_t s) {
+    return launch_interleave_t(p, np, max_cols, s);
 }
 
 hipError_t launch_batch(const BatchPlan<double, double>* p, uint32_t np, uint32_t units, int grid, int threads,

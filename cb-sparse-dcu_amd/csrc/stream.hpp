@@ -60,10 +60,18 @@ struct BatchPlan {
     const A* x;               // column-major n x nrhs
     A* y;                     // column-major m x nrhs
     uint64_t ldx, ldy;
+    A* xg;                    // nrhs > 1: x interleaved in groups of 4 right-hand sides, 4 n per group
     A* partial;               // nrhs x partial_stride
     uint32_t* counters;       // nrhs x counter_stride (zero between launches)
     uint64_t partial_stride, counter_stride;
 };
+
+// Fills BatchPlan::xg of every plan with nrhs > 1 from its x (max_cols: the
+// largest groups * n over those plans; 0 launches nothing).  Runs before
+// launch_batch on the same stream.
+hipError_t launch_interleave(const BatchPlan<double, double>* p, uint32_t np, uint32_t max_cols, hipStream_t s);
+hipError_t launch_interleave(const BatchPlan<float, float>* p, uint32_t np, uint32_t max_cols, hipStream_t s);
+hipError_t launch_interleave(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t max_cols, hipStream_t s);
 
 // Resident workgroups per CU of the batched kernel for this shape (0 = unknown).
 int batch_blocks_per_cu(int dtype, bool comp, int threads, size_t lds);
