@@ -25,6 +25,9 @@
 namespace cbs {
 constexpr int kBatchMaxRhs = 16;
 constexpr int kMI355XCUs = 256;
+// autotuner: copies of a candidate per timed batch (work units to fill)
+constexpr uint32_t kAutotuneUnits = 2048;
+constexpr uint32_t kAutotuneMaxCopies = 64;
 }  // namespace cbs
 
 struct cbspmv_batch {
@@ -37,6 +40,11 @@ struct cbspmv_batch {
     int grid = 8;
     cbs::DeviceBuf desc, partial, counters;
     std::vector<uint8_t> host_desc;   // last uploaded descriptor bytes
+    // stream entries per tile of every plan (the unit cost estimate), and
+    // the per-XCD unit queue counters (zeroed by the prologue kernel)
+    std::vector<std::vector<uint32_t>> tile_entries;
+    cbs::DeviceBuf queue;
+    cbs::XcdSplit split{}, graph_split{};
     // x interleaved per group of right-hand sides (plans with several) and
     // the largest groups * n, for eager calls
     cbs::DeviceBuf xg;
@@ -75,6 +83,39 @@ void fill_args(const cbspmv_plan& p, StreamArgs<V, A>& a) {
 }
 
 constexpr uint32_t kRhsGroup = 4;   // right-hand sides per interleaved group (stream.hip kMultiRhs)
+
+// Estimated cost of one work unit, in stream entries: a tile with one
+// right-hand side streams its entries once with its LDS x cache; with
+// several, once per group of kRhsGroup, gathering every entry's x values
+// (about 1.5x the one-RHS cost per entry); plus a fixed cost per unit.
+double unit_cost(uint32_t entries, int nrhs) {
+    constexpr double kUnitFixed = 2048.0, kMultiWeight = 1.5;
+    if (nrhs <= 0) return kUnitFixed;
+    if (nrhs == 1) return kUnitFixed + entries;
+    return kUnitFixed + kMultiWeight * entries * (double)(((uint32_t)nrhs + kRhsGroup - 1) / kRhsGroup);
+}
+
+// Contiguous unit ranges of (about) equal estimated cost, one per XCD.
+XcdSplit split_units(const cbspmv_batch& b, const int* nrhs) {
+    std::vector<double> cost;
+    cost.reserve(b.units);
+    for (size_t i = 0; i < b.plans.size(); ++i) {
+        const std::vector<uint32_t>& te = b.tile_entries[i];
+        if (te.empty()) cost.push_back(unit_cost(0, nrhs[i]));
+        for (uint32_t e : te) cost.push_back(unit_cost(e, nrhs[i]));
+    }
+    double total = 0.0;
+    for (double c : cost) total += c;
+    XcdSplit sp{};
+    double acc = 0.0;
+    uint32_t u = 0, k = 1;
+    for (; u < (uint32_t)cost.size() && k < (uint32_t)kXcds; ++u) {
+        acc += cost[u];
+        while (k < (uint32_t)kXcds && acc >= total * k / kXcds) sp.begin[k++] = u + 1;
+    }
+    for (; k <= (uint32_t)kXcds; ++k) sp.begin[k] = (uint32_t)cost.size();
+    return sp;
+}
 
 // Descriptor bytes for one call (x/y pointer lists and RHS counts).  Plans
 // with several right-hand sides get a slice of xg (grown here when too
@@ -135,10 +176,13 @@ int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* 
 }
 
 int build_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* nrhs, DeviceBuf& xg,
-               uint32_t& max_cols, std::vector<uint8_t>& out, hipStream_t s) {
-    if (b.dtype == CBSPMV_F64) return make_desc<double, double>(b, x, y, nrhs, xg, max_cols, out, s);
-    if (b.dtype == CBSPMV_F32) return make_desc<float, float>(b, x, y, nrhs, xg, max_cols, out, s);
-    return make_desc<dev::bf16_t, float>(b, x, y, nrhs, xg, max_cols, out, s);
+               uint32_t& max_cols, XcdSplit& split, std::vector<uint8_t>& out, hipStream_t s) {
+    int rc;
+    if (b.dtype == CBSPMV_F64) rc = make_desc<double, double>(b, x, y, nrhs, xg, max_cols, out, s);
+    else if (b.dtype == CBSPMV_F32) rc = make_desc<float, float>(b, x, y, nrhs, xg, max_cols, out, s);
+    else rc = make_desc<dev::bf16_t, float>(b, x, y, nrhs, xg, max_cols, out, s);
+    if (rc == CBSPMV_OK) split = split_units(b, nrhs);
+    return rc;
 }
 
 int upload_desc(DeviceBuf& buf, const std::vector<uint8_t>& bytes, hipStream_t s) {
@@ -153,24 +197,25 @@ int upload_desc(DeviceBuf& buf, const std::vector<uint8_t>& bytes, hipStream_t s
     return CBSPMV_OK;
 }
 
-// The interleave launch (plans with several right-hand sides), then the
-// batched launch, on stream s.
-int launch(cbspmv_batch& b, const void* desc, uint32_t xg_cols, hipStream_t s) {
+// The prologue (queue reset, x interleave), then the batched launch, on
+// stream s.
+int launch(cbspmv_batch& b, const void* desc, uint32_t xg_cols, const XcdSplit& split, hipStream_t s) {
     if (b.plans.empty()) return CBSPMV_OK;
     hipError_t e;
     const uint32_t np = (uint32_t)b.plans.size();
+    uint32_t* q = static_cast<uint32_t*>(b.queue.ptr);
     if (b.dtype == CBSPMV_F64) {
         const auto* d = static_cast<const BatchPlan<double, double>*>(desc);
-        e = launch_interleave(d, np, xg_cols, s);
-        if (e == hipSuccess) e = launch_batch(d, np, b.units, b.grid, b.threads, b.lds, false, s);
+        e = launch_prologue(d, np, xg_cols, q, s);
+        if (e == hipSuccess) e = launch_batch(d, np, split, q, b.grid, b.threads, b.lds, false, s);
     } else if (b.dtype == CBSPMV_F32) {
         const auto* d = static_cast<const BatchPlan<float, float>*>(desc);
-        e = launch_interleave(d, np, xg_cols, s);
-        if (e == hipSuccess) e = launch_batch(d, np, b.units, b.grid, b.threads, b.lds, b.comp, s);
+        e = launch_prologue(d, np, xg_cols, q, s);
+        if (e == hipSuccess) e = launch_batch(d, np, split, q, b.grid, b.threads, b.lds, b.comp, s);
     } else {
         const auto* d = static_cast<const BatchPlan<dev::bf16_t, float>*>(desc);
-        e = launch_interleave(d, np, xg_cols, s);
-        if (e == hipSuccess) e = launch_batch(d, np, b.units, b.grid, b.threads, b.lds, b.comp, s);
+        e = launch_prologue(d, np, xg_cols, q, s);
+        if (e == hipSuccess) e = launch_batch(d, np, split, q, b.grid, b.threads, b.lds, b.comp, s);
     }
     return e == hipSuccess ? CBSPMV_OK : hip_fail(e, "batched launch");
 }
@@ -218,6 +263,13 @@ int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** bat
         if (rc) { delete b; return rc; }
         b->plans.push_back(p);
         b->units += std::max<uint32_t>(1u, (uint32_t)(h.tiles.size() / 2));
+        {
+            std::vector<uint32_t> te(h.tiles.size() / 2, 0u);
+            const size_t ti = (size_t)h.o.tile_items;
+            for (size_t it = 0; it < (size_t)h.tile_items_count; ++it)
+                te[it / ti] += h.items[it].e_end - h.items[it].e_begin;
+            b->tile_entries.push_back(std::move(te));
+        }
         partial_elems += (size_t)h.partial_slots * cbs::kBatchMaxRhs * 2;   // room for (hi, lo)
         counter_elems += h.splits.size() * cbs::kBatchMaxRhs;
         // waves per workgroup: a tile with more items than waves deals them round-robin
@@ -236,7 +288,8 @@ int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** bat
     const uint32_t cap = (uint32_t)(cbs::kMI355XCUs * per_cu);
     b->grid = (int)((std::min<uint32_t>(std::max<uint32_t>(b->units, 1u), cap) + 7u) / 8u * 8u);
     int rc;
-    if ((rc = cbs::upload_bytes(b->partial, nullptr, std::max<size_t>(partial_elems, 1) * eb, s)) ||
+    if ((rc = cbs::upload_bytes(b->queue, nullptr, cbs::kXcds * sizeof(uint32_t), s)) ||
+        (rc = cbs::upload_bytes(b->partial, nullptr, std::max<size_t>(partial_elems, 1) * eb, s)) ||
         (rc = cbs::upload_bytes(b->counters, nullptr, std::max<size_t>(counter_elems, 1) * 4, s))) {
         cbspmv_batch_destroy(b);
         return rc;
@@ -256,13 +309,13 @@ int cbspmv_batch_spmv(cbspmv_batch* batch, const void* const* d_x, void* const* 
     if (!d_x || !d_y || !nrhs) return fail(CBSPMV_ERR_ARG, "null argument");
     const hipStream_t s = static_cast<hipStream_t>(stream);
     std::vector<uint8_t> bytes;
-    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, batch->xg, batch->xg_cols, bytes, s);
+    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, batch->xg, batch->xg_cols, batch->split, bytes, s);
     if (rc) return rc;
     if (bytes != batch->host_desc) {
         if ((rc = cbs::upload_desc(batch->desc, bytes, s))) return rc;
         batch->host_desc.swap(bytes);
     }
-    return cbs::launch(*batch, batch->desc.ptr, batch->xg_cols, s);
+    return cbs::launch(*batch, batch->desc.ptr, batch->xg_cols, batch->split, s);
 }
 
 int cbspmv_batch_capture(cbspmv_batch* batch, const void* const* d_x, void* const* d_y, const int* nrhs,
@@ -273,11 +326,12 @@ int cbspmv_batch_capture(cbspmv_batch* batch, const void* const* d_x, void* cons
     const hipStream_t s = static_cast<hipStream_t>(stream);
     cbs::drop_graph(*batch);   // before graph_xg may be reallocated
     std::vector<uint8_t> bytes;
-    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, batch->graph_xg, batch->graph_xg_cols, bytes, s);
+    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, batch->graph_xg, batch->graph_xg_cols, batch->graph_split, bytes,
+                             s);
     if (rc) return rc;
     if ((rc = cbs::upload_desc(batch->graph_desc, bytes, s))) return rc;
     CBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    rc = cbs::launch(*batch, batch->graph_desc.ptr, batch->graph_xg_cols, s);
+    rc = cbs::launch(*batch, batch->graph_desc.ptr, batch->graph_xg_cols, batch->graph_split, s);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -299,7 +353,7 @@ void cbspmv_batch_destroy(cbspmv_batch* batch) {
     if (!batch) return;
     cbs::drop_graph(*batch);
     for (cbs::DeviceBuf* d : {&batch->desc, &batch->partial, &batch->counters, &batch->graph_desc, &batch->xg,
-                              &batch->graph_xg})
+                              &batch->graph_xg, &batch->queue})
         cbs::release(*d);
     delete batch;
 }
@@ -310,8 +364,8 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
     if (!best) return fail(CBSPMV_ERR_ARG, "null output");
     *best = nullptr;
     if (nrhs < 1 || nrhs > cbs::kBatchMaxRhs) return fail(CBSPMV_ERR_ARG, "nrhs must be in 1..16");
-    static const int kDefault[] = {16, 4, 16, 9, 32, 4, 32, 9};
-    if (!candidates) { candidates = kDefault; ncandidates = 4; }
+    static const int kDefault[] = {16, 0, 32, 0, 16, 4, 16, 9, 32, 4, 32, 9};
+    if (!candidates) { candidates = kDefault; ncandidates = 6; }
     if (ncandidates < 1) return fail(CBSPMV_ERR_ARG, "no candidates");
     cbspmv_opts o;
     if (base) o = *base;
@@ -344,13 +398,37 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
     for (int c = 0; c < ncandidates && rc == CBSPMV_OK; ++c) {
         cbspmv_opts oc = o;
         oc.block_size = candidates[2 * c];
-        oc.waves_per_block = candidates[2 * c + 1];
-        oc.stream_tile_waves = candidates[2 * c + 1];
+        if (candidates[2 * c + 1] > 0) {
+            oc.waves_per_block = candidates[2 * c + 1];
+            oc.stream_tile_waves = candidates[2 * c + 1];
+        }
         cbspmv_plan* p = nullptr;
         rc = cbspmv_build(m, n, nnz, row, col, val, &oc, &p);
         if (rc) break;
         if ((rc = cbs::upload_plan(*p, s))) { cbspmv_destroy(p); break; }
+        // Timed the way a batch runs it: K copies of the plan in one batch
+        // (K sized so that the copies' work units fill the GPU several
+        // times over), costed at 1/K of that launch, i.e. its throughput
+        // beside other matrices rather than its latency alone on the GPU.
+        // Plans a batch does not take: one launch per right-hand side.
+        const uint32_t units = std::max<uint32_t>(1u, (uint32_t)(p->host.tiles.size() / 2));
+        const int K = (int)std::min<uint32_t>(cbs::kAutotuneMaxCopies, (cbs::kAutotuneUnits + units - 1) / units);
+        cbspmv_batch* one = nullptr;
+        void* dyk = nullptr;
+        std::vector<const void*> bx((size_t)K, dx);
+        std::vector<void*> by((size_t)K, nullptr);
+        std::vector<int> bn((size_t)K, nrhs);
+        if (hipMalloc(&dyk, std::max<size_t>(1, (size_t)K * m * nrhs * eb)) == hipSuccess) {
+            for (int k = 0; k < K; ++k) by[k] = static_cast<uint8_t*>(dyk) + (size_t)k * m * nrhs * eb;
+            std::vector<cbspmv_plan*> copies((size_t)K, p);
+            if (cbspmv_batch_create(copies.data(), K, &one, stream) != CBSPMV_OK) one = nullptr;
+        }
+        const double per_run = one ? (double)K : 1.0;
         auto run = [&]() {
+            if (one) {
+                if (rc == CBSPMV_OK) rc = cbspmv_batch_spmv(one, bx.data(), by.data(), bn.data(), stream);
+                return;
+            }
             for (int r = 0; r < nrhs && rc == CBSPMV_OK; ++r)
                 rc = cbs::run_spmv(*p, static_cast<uint8_t*>(dx) + (size_t)r * n * eb,
                                    static_cast<uint8_t*>(dy) + (size_t)r * m * eb, s);
@@ -363,7 +441,9 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
         (void)hipEventSynchronize(e1);
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        const double t = ms / kReps;
+        const double t = ms / kReps / per_run;
+        if (one) cbspmv_batch_destroy(one);
+        if (dyk) (void)hipFree(dyk);
         if (rc == CBSPMV_OK && t < best_t) {
             if (*best) cbspmv_destroy(*best);
             *best = p;

@@ -847,21 +847,27 @@ __global__ __launch_bounds__(256) void panel_sums(A* y, const A* out, const uint
     }
 }
 
-// Batched persistent launch: work units are (plan, tile) pairs, dealt to
-// workgroups per XCD in contiguous ranges; a unit runs its tile for every
-// right-hand side of its plan in turn (the tile's slab is then re-read from
-// L2).  Unit 0 of a plan also zeroes the plan's empty rows.
+// Batched persistent launch: work units are (plan, tile) pairs.  Each XCD
+// owns the contiguous unit range split.begin[xcd] .. split.begin[xcd + 1]
+// (ranges of equal estimated cost, so a plan's tiles share one L2), and its
+// workgroups take the next unit of that range from queue[xcd] (zeroed by
+// batch_prologue before the launch).  A unit runs its tile for every
+// right-hand side of its plan in turn.  Unit 0 of a plan also zeroes the
+// plan's empty rows.
 template <typename V, typename A, bool COMP, int EPL>
 __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan<V, A>* plans, uint32_t nplans,
-                                                                  uint32_t units) {
+                                                                  XcdSplit split, uint32_t* queue) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ uint32_t next_unit;
     A* xs = reinterpret_cast<A*>(lds_raw);
-    const uint32_t groups = gridDim.x >> 3;   // workgroups per XCD (grid is a multiple of 8)
-    const uint32_t xcd = blockIdx.x & 7u, rank = blockIdx.x >> 3;
-    const uint32_t per = (units + 7u) >> 3;
-    for (uint32_t v = rank; v < per; v += groups) {
-        const uint32_t u = xcd * per + v;
-        if (u >= units) break;
+    const uint32_t xcd = blockIdx.x & 7u;   // the hardware deals workgroups round-robin over the 8 XCDs
+    const uint32_t u1 = split.begin[xcd + 1];
+    for (;;) {
+        if (threadIdx.x == 0) next_unit = split.begin[xcd] + atomicAdd(queue + xcd, 1u);
+        __syncthreads();
+        const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)next_unit);
+        __syncthreads();   // next_unit is rewritten at the next unit
+        if (u >= u1) break;
         // plan of unit u: last plan with unit_begin <= u (binary search)
         uint32_t lo = 0, hi = nplans;
         while (hi - lo > 1) {
@@ -901,8 +907,8 @@ __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan
     }
 }
 
-// x of every plan with several right-hand sides, interleaved in groups of
-// kMultiRhs for stream_tile_multi: xg[(g*n + col)*kMultiRhs + j] =
+// batch_prologue writes the x of every plan with several right-hand sides
+// interleaved in groups of kMultiRhs for stream_tile_multi: xg[(g*n + col)*kMultiRhs + j] =
 // x[(g*kMultiRhs + j)*ldx + col], 0 past the last right-hand side.  Reads
 // are coalesced over col, each thread writes one 16-byte (FP32) or two
 // 16-byte (FP64) vectors.
@@ -915,8 +921,12 @@ __device__ __forceinline__ void store_rhs(double* p, const double (&v)[4]) {
 }
 static_assert(kMultiRhs == 4, "store_rhs writes groups of 4");
 
+// Runs before every stream_batch launch: zeroes the per-XCD unit queues
+// and interleaves x.
 template <typename V, typename A>
-__global__ __launch_bounds__(256) void interleave_rhs(const BatchPlan<V, A>* plans, uint32_t nplans) {
+__global__ __launch_bounds__(256) void batch_prologue(const BatchPlan<V, A>* plans, uint32_t nplans,
+                                                      uint32_t* queue) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < kXcds) queue[threadIdx.x] = 0;
     for (uint32_t p = blockIdx.y; p < nplans; p += gridDim.y) {
         const BatchPlan<V, A>& bp = plans[p];
         if (bp.nrhs < 2) continue;
@@ -967,8 +977,8 @@ hipError_t launch_stream_any(const StreamArgs<V, A>& a, int grid, bool comp, boo
 }
 
 template <typename V, typename A, bool COMP>
-hipError_t launch_batch_t(const BatchPlan<V, A>* plans, uint32_t nplans, uint32_t units, int grid, int threads,
-                          size_t lds, hipStream_t s) {
+hipError_t launch_batch_t(const BatchPlan<V, A>* plans, uint32_t nplans, const XcdSplit& split, uint32_t* queue,
+                          int grid, int threads, size_t lds, hipStream_t s) {
     constexpr int EPL = sizeof(V) >= 8 ? 4 : 8;
     auto kern = dev::stream_batch<V, A, COMP, EPL>;
     static bool attr_set = false;
@@ -977,7 +987,7 @@ hipError_t launch_batch_t(const BatchPlan<V, A>* plans, uint32_t nplans, uint32_
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, s, plans, nplans, units);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, s, plans, nplans, split, queue);
     return hipGetLastError();
 }
 
@@ -996,36 +1006,42 @@ int batch_blocks_per_cu(int dtype, bool comp, int threads, size_t lds) {
 }
 
 template <typename V, typename A>
-hipError_t launch_interleave_t(const BatchPlan<V, A>* p, uint32_t np, uint32_t max_cols, hipStream_t s) {
-    if (!np || !max_cols) return hipSuccess;
-    const dim3 grid(std::min<uint32_t>(32u, (max_cols + 255u) / 256u), std::min<uint32_t>(np, 65535u));
-    hipLaunchKernelGGL((dev::interleave_rhs<V, A>), grid, dim3(256), 0, s, p, np);
+hipError_t launch_prologue_t(const BatchPlan<V, A>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
+                             hipStream_t s) {
+    // one column per thread for plans up to 64K columns, so that few loads
+    // per thread are in flight one after another
+    const dim3 grid(std::max(1u, std::min<uint32_t>(256u, (max_cols + 255u) / 256u)),
+                    max_cols ? std::min<uint32_t>(std::max(np, 1u), 65535u) : 1u);
+    hipLaunchKernelGGL((dev::batch_prologue<V, A>), grid, dim3(256), 0, s, p, max_cols ? np : 0u, queue);
     return hipGetLastError();
 }
-hipError_t launch_interleave(const BatchPlan<double, double>* p, uint32_t np, uint32_t max_cols, hipStream_t s) {
-    return launch_interleave_t(p, np, max_cols, s);
+hipError_t launch_prologue(const BatchPlan<double, double>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
+                           hipStream_t s) {
+    return launch_prologue_t(p, np, max_cols, queue, s);
 }
-hipError_t launch_interleave(const BatchPlan<float, float>* p, uint32_t np, uint32_t max_cols, hipStream_t s) {
-    return launch_interleave_t(p, np, max_cols, s);
+hipError_t launch_prologue(const BatchPlan<float, float>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
+                           hipStream_t s) {
+    return launch_prologue_t(p, np, max_cols, queue, s);
 }
-hipError_t launch_interleave(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t max_cols, hipStream_t s) {
-    return launch_interleave_t(p, np, max_cols, s);
+hipError_t launch_prologue(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
+                           hipStream_t s) {
+    return launch_prologue_t(p, np, max_cols, queue, s);
 }
 
-hipError_t launch_batch(const BatchPlan<double, double>* p, uint32_t np, uint32_t units, int grid, int threads,
-                        size_t lds, bool comp, hipStream_t s) {
+hipError_t launch_batch(const BatchPlan<double, double>* p, uint32_t np, const XcdSplit& split, uint32_t* queue,
+                        int grid, int threads, size_t lds, bool comp, hipStream_t s) {
     (void)comp;
-    return launch_batch_t<double, double, false>(p, np, units, grid, threads, lds, s);
+    return launch_batch_t<double, double, false>(p, np, split, queue, grid, threads, lds, s);
 }
-hipError_t launch_batch(const BatchPlan<float, float>* p, uint32_t np, uint32_t units, int grid, int threads,
-                        size_t lds, bool comp, hipStream_t s) {
-    return comp ? launch_batch_t<float, float, true>(p, np, units, grid, threads, lds, s)
-                : launch_batch_t<float, float, false>(p, np, units, grid, threads, lds, s);
+hipError_t launch_batch(const BatchPlan<float, float>* p, uint32_t np, const XcdSplit& split, uint32_t* queue,
+                        int grid, int threads, size_t lds, bool comp, hipStream_t s) {
+    return comp ? launch_batch_t<float, float, true>(p, np, split, queue, grid, threads, lds, s)
+                : launch_batch_t<float, float, false>(p, np, split, queue, grid, threads, lds, s);
 }
-hipError_t launch_batch(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t units, int grid, int threads,
-                        size_t lds, bool comp, hipStream_t s) {
-    return comp ? launch_batch_t<dev::bf16_t, float, true>(p, np, units, grid, threads, lds, s)
-                : launch_batch_t<dev::bf16_t, float, false>(p, np, units, grid, threads, lds, s);
+hipError_t launch_batch(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, const XcdSplit& split, uint32_t* queue,
+                        int grid, int threads, size_t lds, bool comp, hipStream_t s) {
+    return comp ? launch_batch_t<dev::bf16_t, float, true>(p, np, split, queue, grid, threads, lds, s)
+                : launch_batch_t<dev::bf16_t, float, false>(p, np, split, queue, grid, threads, lds, s);
 }
 
 template <typename A>

@@ -66,21 +66,31 @@ struct BatchPlan {
     uint64_t partial_stride, counter_stride;
 };
 
-// Fills BatchPlan::xg of every plan with nrhs > 1 from its x (max_cols: the
-// largest groups * n over those plans; 0 launches nothing).  Runs before
-// launch_batch on the same stream.
-hipError_t launch_interleave(const BatchPlan<double, double>* p, uint32_t np, uint32_t max_cols, hipStream_t s);
-hipError_t launch_interleave(const BatchPlan<float, float>* p, uint32_t np, uint32_t max_cols, hipStream_t s);
-hipError_t launch_interleave(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t max_cols, hipStream_t s);
+// The work units (plan, tile) of XCD k in a batched launch:
+// [begin[k], begin[k + 1]).
+struct XcdSplit {
+    uint32_t begin[kXcds + 1];
+};
+
+// Runs before every launch_batch, on the same stream: zeroes the kXcds unit
+// queue counters and fills BatchPlan::xg of every plan with nrhs > 1 from
+// its x (max_cols: the largest groups * n over those plans, 0 if none).
+hipError_t launch_prologue(const BatchPlan<double, double>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
+                           hipStream_t s);
+hipError_t launch_prologue(const BatchPlan<float, float>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
+                           hipStream_t s);
+hipError_t launch_prologue(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
+                           hipStream_t s);
 
 // Resident workgroups per CU of the batched kernel for this shape (0 = unknown).
 int batch_blocks_per_cu(int dtype, bool comp, int threads, size_t lds);
-hipError_t launch_batch(const BatchPlan<double, double>* p, uint32_t np, uint32_t units, int grid, int threads,
-                        size_t lds, bool comp, hipStream_t s);
-hipError_t launch_batch(const BatchPlan<float, float>* p, uint32_t np, uint32_t units, int grid, int threads,
-                        size_t lds, bool comp, hipStream_t s);
-hipError_t launch_batch(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t units, int grid, int threads,
-                        size_t lds, bool comp, hipStream_t s);
+// grid: a multiple of kXcds
+hipError_t launch_batch(const BatchPlan<double, double>* p, uint32_t np, const XcdSplit& split, uint32_t* queue,
+                        int grid, int threads, size_t lds, bool comp, hipStream_t s);
+hipError_t launch_batch(const BatchPlan<float, float>* p, uint32_t np, const XcdSplit& split, uint32_t* queue,
+                        int grid, int threads, size_t lds, bool comp, hipStream_t s);
+hipError_t launch_batch(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, const XcdSplit& split, uint32_t* queue,
+                        int grid, int threads, size_t lds, bool comp, hipStream_t s);
 
 // y[r] += the panel sums of row r, for every row with panel entries (after
 // a stream launch of a plan with column panels, on the same stream).

@@ -227,11 +227,16 @@ int cbspmv_batch_capture(cbspmv_batch* batch, const void* const* d_x, void* cons
 int cbspmv_batch_replay(cbspmv_batch* batch, void* stream);
 void cbspmv_batch_destroy(cbspmv_batch* batch);
 
-/* Autotuner: builds the candidate plans for one COO matrix, times nrhs
- * SpMVs of each on the device (3 warm-up, mean of 20) and returns the
- * fastest, destroying the others.  `candidates` lists (block_size, waves)
- * pairs; waves sets both the blocked schedule's waves per workgroup and the
- * stream schedule's tile waves.  NULL = {16,4},{16,9},{32,4},{32,9}.
+/* Autotuner: builds the candidate plans for one COO matrix, times each on
+ * the device in a batch of copies of itself with nrhs right-hand sides
+ * each (enough copies to fill the GPU; 3 warm-up, mean of 20; the time is
+ * per copy, i.e. what the matrix costs inside a full batch; a plan a batch
+ * does not take is timed with nrhs single launches) and returns the
+ * fastest, destroying the others.  *best_ms is that per-copy time.  `candidates`
+ * lists (block_size, waves) pairs; waves sets both the blocked schedule's
+ * waves per workgroup and the stream schedule's tile waves, 0 keeps both
+ * as `base` has them (its auto tile waves by default).
+ * NULL = {16,0},{32,0},{16,4},{16,9},{32,4},{32,9}.
  * Allocates device memory for its test vectors. */
 int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const double* val,
                     const cbspmv_opts* base, const int* candidates, int ncandidates,

@@ -142,8 +142,19 @@ def test_autotune_returns_a_correct_plan(gpu):
     p, ms = autotune(m, n, r, c, v, nrhs=2, dtype="f32")
     assert ms > 0
     waves, _ = p.export("stream_tile_shape")
-    assert int(waves) in (4, 9)
+    assert int(waves) in (4, 9, 16)   # 16: the auto shape of FP32 plans
     x = np.random.default_rng(6).uniform(-1, 1, n)
     y = p.spmv(torch.tensor(x, dtype=torch.float32, device=gpu)).cpu().numpy().astype(np.float64)
     assert pyoracle.rel_l2(y, pyoracle.spmv_seq(m, r, c, v, x.astype(np.float32).astype(np.float64))) <= 1e-6
+    p.close()
+    # explicit candidates are honoured, and the plan runs in a batch
+    p, ms = autotune(m, n, r, c, v, nrhs=5, candidates=[(32, 9)], dtype="f32")
+    assert int(p.export("stream_tile_shape")[0]) == 9
+    X = torch.tensor(np.random.default_rng(7).uniform(-1, 1, (5, n)), dtype=torch.float32, device=gpu)
+    Y = torch.zeros((5, m), dtype=torch.float32, device=gpu)
+    b = Batch([p])
+    b.spmv([X], [Y])
+    torch.cuda.synchronize()
+    _check([(m, n, r, c, v)], [X], [Y], 1e-6)
+    b.close()
     p.close()
