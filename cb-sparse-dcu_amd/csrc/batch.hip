@@ -28,6 +28,7 @@ constexpr int kMI355XCUs = 256;
 // autotuner: copies of a candidate per timed batch (work units to fill)
 constexpr uint32_t kAutotuneUnits = 2048;
 constexpr uint32_t kAutotuneMaxCopies = 64;
+constexpr size_t kAutotuneYBytes = size_t(1) << 30;   // bound on the copies' y
 }  // namespace cbs
 
 struct cbspmv_batch {
@@ -412,14 +413,17 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
         // beside other matrices rather than its latency alone on the GPU.
         // Plans a batch does not take: one launch per right-hand side.
         const uint32_t units = std::max<uint32_t>(1u, (uint32_t)(p->host.tiles.size() / 2));
-        const int K = (int)std::min<uint32_t>(cbs::kAutotuneMaxCopies, (cbs::kAutotuneUnits + units - 1) / units);
+        const size_t y_bytes = std::max<size_t>(1, (size_t)m * nrhs * eb);
+        const int K = (int)std::max<size_t>(
+            1, std::min<size_t>({(size_t)cbs::kAutotuneMaxCopies, (cbs::kAutotuneUnits + units - 1) / units,
+                                 cbs::kAutotuneYBytes / y_bytes}));
         cbspmv_batch* one = nullptr;
         void* dyk = nullptr;
         std::vector<const void*> bx((size_t)K, dx);
         std::vector<void*> by((size_t)K, nullptr);
         std::vector<int> bn((size_t)K, nrhs);
-        if (hipMalloc(&dyk, std::max<size_t>(1, (size_t)K * m * nrhs * eb)) == hipSuccess) {
-            for (int k = 0; k < K; ++k) by[k] = static_cast<uint8_t*>(dyk) + (size_t)k * m * nrhs * eb;
+        if (hipMalloc(&dyk, (size_t)K * y_bytes) == hipSuccess) {
+            for (int k = 0; k < K; ++k) by[k] = static_cast<uint8_t*>(dyk) + (size_t)k * y_bytes;
             std::vector<cbspmv_plan*> copies((size_t)K, p);
             if (cbspmv_batch_create(copies.data(), K, &one, stream) != CBSPMV_OK) one = nullptr;
         }

@@ -134,6 +134,59 @@ def test_batch_graph_replay(gpu):
 
 
 @pytest.mark.gpu
+def test_batch_buffers_follow_rhs_counts(gpu):
+    """The interleaved-x buffers grow with the right-hand-side counts of a
+    call; a captured graph keeps its own, so eager calls with more
+    right-hand sides after a capture leave its replays correct.  The same
+    plan may appear several times in one batch (the autotuner times plans
+    that way), each copy with its own vectors."""
+    import torch
+
+    mats = _mats(seed=5, count=3)[:3]
+    plans = [Plan(m, n, r, c, v, dtype="f32") for (m, n, r, c, v) in mats]
+    rng = np.random.default_rng(8)
+
+    def vecs(k, plans_, mats_):
+        X = [torch.tensor(rng.uniform(-1, 1, (k, p.n)), dtype=torch.float32, device=gpu) for p in plans_]
+        Y = [torch.full((k, p.m), 3.0, dtype=torch.float32, device=gpu) for p in plans_]
+        return X, Y
+
+    b = Batch(plans)
+    for k in (2, 5):   # eager: the buffer grows between calls
+        X, Y = vecs(k, plans, mats)
+        b.spmv(X, Y)
+        torch.cuda.synchronize()
+        _check(mats, X, Y, 1e-6)
+    s = torch.cuda.Stream()
+    Xg, Yg = vecs(3, plans, mats)
+    with torch.cuda.stream(s):
+        b.capture(Xg, Yg, s)
+    s.synchronize()
+    _check(mats, Xg, Yg, 1e-6)
+    X, Y = vecs(13, plans, mats)   # eager call after the capture, more right-hand sides
+    b.spmv(X, Y)
+    torch.cuda.synchronize()
+    _check(mats, X, Y, 1e-6)
+    for y in Yg:
+        y.fill_(-1.0)
+    torch.cuda.synchronize()
+    b.replay(s)
+    s.synchronize()
+    _check(mats, Xg, Yg, 1e-6)
+    b.close()
+
+    same = [plans[0]] * 3
+    X, Y = vecs(6, same, [mats[0]] * 3)
+    b = Batch(same)
+    b.spmv(X, Y)
+    torch.cuda.synchronize()
+    _check([mats[0]] * 3, X, Y, 1e-6)
+    b.close()
+    for p in plans:
+        p.close()
+
+
+@pytest.mark.gpu
 def test_autotune_returns_a_correct_plan(gpu):
     import torch
 
