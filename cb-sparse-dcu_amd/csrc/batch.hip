@@ -47,12 +47,12 @@ struct cbspmv_batch {
     cbs::DeviceBuf queue;
     cbs::XcdSplit split{}, graph_split{};
     // x interleaved per group of right-hand sides (plans with several) and
-    // the largest groups * n, for eager calls
+    // the interleave's block count, for eager calls
     cbs::DeviceBuf xg;
-    uint32_t xg_cols = 0;
+    uint32_t xg_blocks = 0;
     // captured graph, the descriptor and interleave buffer it reads
     cbs::DeviceBuf graph_desc, graph_xg;
-    uint32_t graph_xg_cols = 0;
+    uint32_t graph_xg_blocks = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -118,25 +118,29 @@ XcdSplit split_units(const cbspmv_batch& b, const int* nrhs) {
     return sp;
 }
 
-// Descriptor bytes for one call (x/y pointer lists and RHS counts).  Plans
-// with several right-hand sides get a slice of xg (grown here when too
-// small) for their interleaved x; max_cols = the largest groups * n.
+// Descriptor bytes for one call: the BatchPlan array (x/y pointers, RHS
+// counts), then the interleave's block -> plan table (xg_blocks entries).
+// Plans with several right-hand sides get a slice of xg (grown here when
+// too small) for their interleaved x.
 template <typename V, typename A>
 int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* nrhs, DeviceBuf& xg,
-              uint32_t& max_cols, std::vector<uint8_t>& out, hipStream_t s) {
+              uint32_t& xg_blocks, std::vector<uint8_t>& out, hipStream_t s) {
     const size_t np = b.plans.size();
     size_t xg_elems = 0;
-    max_cols = 0;
+    uint64_t cols_sum = 0;
     for (size_t i = 0; i < np; ++i) {
         const Plan& h = b.plans[i]->host;
         if (nrhs[i] < 0 || nrhs[i] > kBatchMaxRhs) return fail(CBSPMV_ERR_ARG, "nrhs must be in 0..16");
         if (nrhs[i] > 0 && ((h.m > 0 && !y[i]) || (h.nnz > 0 && !x[i]))) return fail(CBSPMV_ERR_ARG, "null vector");
         if (nrhs[i] > 1) {
-            const uint32_t cols = ((uint32_t)nrhs[i] + kRhsGroup - 1) / kRhsGroup * (uint32_t)h.n;
+            const uint64_t cols = ((uint64_t)nrhs[i] + kRhsGroup - 1) / kRhsGroup * (uint64_t)h.n;
             xg_elems += (size_t)cols * kRhsGroup;
-            max_cols = std::max(max_cols, cols);
+            cols_sum += cols;
         }
     }
+    if (cols_sum >= (1ull << 31))
+        return fail(CBSPMV_ERR_UNSUPPORTED, "batch: sum over plans of n x ceil(nrhs / 4) must stay below 2^31");
+    std::vector<uint32_t> block_plan;
     if (xg.bytes < xg_elems * sizeof(A)) {
         const int rc = upload_bytes(xg, nullptr, xg_elems * sizeof(A), s);
         if (rc) return rc;
@@ -144,7 +148,7 @@ int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* 
     std::vector<BatchPlan<V, A>> d(np);
     A* part = static_cast<A*>(b.partial.ptr);
     uint32_t* cnt = static_cast<uint32_t*>(b.counters.ptr);
-    A* xgp = static_cast<A*>(xg.ptr);
+    size_t xg_off = 0;   // elements of xg handed out so far
     uint32_t unit = 0;
     for (size_t i = 0; i < np; ++i) {
         const cbspmv_plan& p = *b.plans[i];
@@ -164,24 +168,30 @@ int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* 
         e.partial = part;
         e.counters = cnt;
         e.xg = nullptr;
+        e.xg_block0 = (uint32_t)block_plan.size();
         if (nrhs[i] > 1) {
-            e.xg = xgp;
-            xgp += (size_t)((uint32_t)nrhs[i] + kRhsGroup - 1) / kRhsGroup * kRhsGroup * (size_t)h.n;
+            const size_t cols = (size_t)((uint32_t)nrhs[i] + kRhsGroup - 1) / kRhsGroup * (size_t)h.n;
+            e.xg = static_cast<A*>(xg.ptr) + xg_off;
+            xg_off += cols * kRhsGroup;
+            block_plan.insert(block_plan.end(), (cols + kPrologueCols - 1) / kPrologueCols, (uint32_t)i);
         }
         part += (size_t)h.partial_slots * parts * kBatchMaxRhs;
         cnt += h.splits.size() * kBatchMaxRhs;
     }
-    out.resize(np * sizeof(BatchPlan<V, A>));
-    if (np) std::memcpy(out.data(), d.data(), out.size());
+    xg_blocks = (uint32_t)block_plan.size();
+    out.resize(np * sizeof(BatchPlan<V, A>) + block_plan.size() * sizeof(uint32_t));
+    if (np) std::memcpy(out.data(), d.data(), np * sizeof(BatchPlan<V, A>));
+    if (!block_plan.empty())
+        std::memcpy(out.data() + np * sizeof(BatchPlan<V, A>), block_plan.data(), block_plan.size() * sizeof(uint32_t));
     return CBSPMV_OK;
 }
 
 int build_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* nrhs, DeviceBuf& xg,
-               uint32_t& max_cols, XcdSplit& split, std::vector<uint8_t>& out, hipStream_t s) {
+               uint32_t& xg_blocks, XcdSplit& split, std::vector<uint8_t>& out, hipStream_t s) {
     int rc;
-    if (b.dtype == CBSPMV_F64) rc = make_desc<double, double>(b, x, y, nrhs, xg, max_cols, out, s);
-    else if (b.dtype == CBSPMV_F32) rc = make_desc<float, float>(b, x, y, nrhs, xg, max_cols, out, s);
-    else rc = make_desc<dev::bf16_t, float>(b, x, y, nrhs, xg, max_cols, out, s);
+    if (b.dtype == CBSPMV_F64) rc = make_desc<double, double>(b, x, y, nrhs, xg, xg_blocks, out, s);
+    else if (b.dtype == CBSPMV_F32) rc = make_desc<float, float>(b, x, y, nrhs, xg, xg_blocks, out, s);
+    else rc = make_desc<dev::bf16_t, float>(b, x, y, nrhs, xg, xg_blocks, out, s);
     if (rc == CBSPMV_OK) split = split_units(b, nrhs);
     return rc;
 }
@@ -200,22 +210,26 @@ int upload_desc(DeviceBuf& buf, const std::vector<uint8_t>& bytes, hipStream_t s
 
 // The prologue (queue reset, x interleave), then the batched launch, on
 // stream s.
-int launch(cbspmv_batch& b, const void* desc, uint32_t xg_cols, const XcdSplit& split, hipStream_t s) {
+int launch(cbspmv_batch& b, const void* desc, uint32_t xg_blocks, const XcdSplit& split, hipStream_t s) {
     if (b.plans.empty()) return CBSPMV_OK;
     hipError_t e;
     const uint32_t np = (uint32_t)b.plans.size();
     uint32_t* q = static_cast<uint32_t*>(b.queue.ptr);
+    // the block -> plan table follows the np BatchPlan entries
+    auto table = [&](size_t plan_bytes) {
+        return reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(desc) + np * plan_bytes);
+    };
     if (b.dtype == CBSPMV_F64) {
         const auto* d = static_cast<const BatchPlan<double, double>*>(desc);
-        e = launch_prologue(d, np, xg_cols, q, s);
+        e = launch_prologue(d, table(sizeof(*d)), xg_blocks, q, s);
         if (e == hipSuccess) e = launch_batch(d, np, split, q, b.grid, b.threads, b.lds, false, s);
     } else if (b.dtype == CBSPMV_F32) {
         const auto* d = static_cast<const BatchPlan<float, float>*>(desc);
-        e = launch_prologue(d, np, xg_cols, q, s);
+        e = launch_prologue(d, table(sizeof(*d)), xg_blocks, q, s);
         if (e == hipSuccess) e = launch_batch(d, np, split, q, b.grid, b.threads, b.lds, b.comp, s);
     } else {
         const auto* d = static_cast<const BatchPlan<dev::bf16_t, float>*>(desc);
-        e = launch_prologue(d, np, xg_cols, q, s);
+        e = launch_prologue(d, table(sizeof(*d)), xg_blocks, q, s);
         if (e == hipSuccess) e = launch_batch(d, np, split, q, b.grid, b.threads, b.lds, b.comp, s);
     }
     return e == hipSuccess ? CBSPMV_OK : hip_fail(e, "batched launch");
@@ -310,13 +324,13 @@ int cbspmv_batch_spmv(cbspmv_batch* batch, const void* const* d_x, void* const* 
     if (!d_x || !d_y || !nrhs) return fail(CBSPMV_ERR_ARG, "null argument");
     const hipStream_t s = static_cast<hipStream_t>(stream);
     std::vector<uint8_t> bytes;
-    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, batch->xg, batch->xg_cols, batch->split, bytes, s);
+    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, batch->xg, batch->xg_blocks, batch->split, bytes, s);
     if (rc) return rc;
     if (bytes != batch->host_desc) {
         if ((rc = cbs::upload_desc(batch->desc, bytes, s))) return rc;
         batch->host_desc.swap(bytes);
     }
-    return cbs::launch(*batch, batch->desc.ptr, batch->xg_cols, batch->split, s);
+    return cbs::launch(*batch, batch->desc.ptr, batch->xg_blocks, batch->split, s);
 }
 
 int cbspmv_batch_capture(cbspmv_batch* batch, const void* const* d_x, void* const* d_y, const int* nrhs,
@@ -327,12 +341,12 @@ int cbspmv_batch_capture(cbspmv_batch* batch, const void* const* d_x, void* cons
     const hipStream_t s = static_cast<hipStream_t>(stream);
     cbs::drop_graph(*batch);   // before graph_xg may be reallocated
     std::vector<uint8_t> bytes;
-    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, batch->graph_xg, batch->graph_xg_cols, batch->graph_split, bytes,
+    int rc = cbs::build_desc(*batch, d_x, d_y, nrhs, batch->graph_xg, batch->graph_xg_blocks, batch->graph_split, bytes,
                              s);
     if (rc) return rc;
     if ((rc = cbs::upload_desc(batch->graph_desc, bytes, s))) return rc;
     CBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    rc = cbs::launch(*batch, batch->graph_desc.ptr, batch->graph_xg_cols, batch->graph_split, s);
+    rc = cbs::launch(*batch, batch->graph_desc.ptr, batch->graph_xg_blocks, batch->graph_split, s);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }

@@ -922,26 +922,31 @@ __device__ __forceinline__ void store_rhs(double* p, const double (&v)[4]) {
 static_assert(kMultiRhs == 4, "store_rhs writes groups of 4");
 
 // Runs before every stream_batch launch: zeroes the per-XCD unit queues
-// and interleaves x.
+// and interleaves x.  Block b works on plan block_plan[b] (a table the host
+// appends to the descriptor), on that plan's interleaved columns
+// (b - xg_block0) * kPrologueCols + [0, kPrologueCols); the plans' column
+// ranges are padded to whole blocks, so every block has one plan.
 template <typename V, typename A>
-__global__ __launch_bounds__(256) void batch_prologue(const BatchPlan<V, A>* plans, uint32_t nplans,
-                                                      uint32_t* queue) {
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < kXcds) queue[threadIdx.x] = 0;
-    for (uint32_t p = blockIdx.y; p < nplans; p += gridDim.y) {
-        const BatchPlan<V, A>& bp = plans[p];
-        if (bp.nrhs < 2) continue;
-        const uint32_t n = (uint32_t)bp.args.n;
-        const uint32_t total = (bp.nrhs + kMultiRhs - 1) / kMultiRhs * n;
-        for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-            const uint32_t g = t / n, col = t - g * n;
-            A v[kMultiRhs];
+__global__ __launch_bounds__(256) void batch_prologue(const BatchPlan<V, A>* plans, const uint32_t* block_plan,
+                                                      uint32_t nblocks, uint32_t* queue) {
+    if (blockIdx.x == 0 && threadIdx.x < kXcds) queue[threadIdx.x] = 0;
+    if (blockIdx.x >= nblocks) return;   // the one block launched when no plan needs x interleaved
+    const BatchPlan<V, A>& bp = plans[block_plan[blockIdx.x]];
+    const uint32_t n = (uint32_t)bp.args.n;
+    const uint32_t cols = (bp.nrhs + kMultiRhs - 1) / kMultiRhs * n;
+    const uint32_t c0 = (blockIdx.x - bp.xg_block0) * kPrologueCols + threadIdx.x;
 #pragma unroll
-            for (int j = 0; j < kMultiRhs; ++j) {
-                const uint32_t r = g * kMultiRhs + (uint32_t)j;
-                v[j] = r < bp.nrhs ? bp.x[(size_t)r * bp.ldx + col] : A(0);
-            }
-            store_rhs(bp.xg + (size_t)t * kMultiRhs, v);
+    for (uint32_t k = 0; k < kPrologueCols / 256; ++k) {
+        const uint32_t c = c0 + k * 256;
+        if (c >= cols) break;
+        const uint32_t g = c / n, col = c - g * n;
+        A v[kMultiRhs];
+#pragma unroll
+        for (int j = 0; j < kMultiRhs; ++j) {
+            const uint32_t r = g * kMultiRhs + (uint32_t)j;
+            v[j] = r < bp.nrhs ? bp.x[(size_t)r * bp.ldx + col] : A(0);
         }
+        store_rhs(bp.xg + (size_t)c * kMultiRhs, v);
     }
 }
 
@@ -1006,26 +1011,23 @@ int batch_blocks_per_cu(int dtype, bool comp, int threads, size_t lds) {
 }
 
 template <typename V, typename A>
-hipError_t launch_prologue_t(const BatchPlan<V, A>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
+hipError_t launch_prologue_t(const BatchPlan<V, A>* p, const uint32_t* block_plan, uint32_t nblocks, uint32_t* queue,
                              hipStream_t s) {
-    // one column per thread for plans up to 64K columns, so that few loads
-    // per thread are in flight one after another
-    const dim3 grid(std::max(1u, std::min<uint32_t>(256u, (max_cols + 255u) / 256u)),
-                    max_cols ? std::min<uint32_t>(std::max(np, 1u), 65535u) : 1u);
-    hipLaunchKernelGGL((dev::batch_prologue<V, A>), grid, dim3(256), 0, s, p, max_cols ? np : 0u, queue);
+    hipLaunchKernelGGL((dev::batch_prologue<V, A>), dim3(std::max(nblocks, 1u)), dim3(256), 0, s, p, block_plan,
+                       nblocks, queue);
     return hipGetLastError();
 }
-hipError_t launch_prologue(const BatchPlan<double, double>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
-                           hipStream_t s) {
-    return launch_prologue_t(p, np, max_cols, queue, s);
+hipError_t launch_prologue(const BatchPlan<double, double>* p, const uint32_t* block_plan, uint32_t nblocks,
+                           uint32_t* queue, hipStream_t s) {
+    return launch_prologue_t(p, block_plan, nblocks, queue, s);
 }
-hipError_t launch_prologue(const BatchPlan<float, float>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
-                           hipStream_t s) {
-    return launch_prologue_t(p, np, max_cols, queue, s);
+hipError_t launch_prologue(const BatchPlan<float, float>* p, const uint32_t* block_plan, uint32_t nblocks,
+                           uint32_t* queue, hipStream_t s) {
+    return launch_prologue_t(p, block_plan, nblocks, queue, s);
 }
-hipError_t launch_prologue(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
-                           hipStream_t s) {
-    return launch_prologue_t(p, np, max_cols, queue, s);
+hipError_t launch_prologue(const BatchPlan<dev::bf16_t, float>* p, const uint32_t* block_plan, uint32_t nblocks,
+                           uint32_t* queue, hipStream_t s) {
+    return launch_prologue_t(p, block_plan, nblocks, queue, s);
 }
 
 hipError_t launch_batch(const BatchPlan<double, double>* p, uint32_t np, const XcdSplit& split, uint32_t* queue,

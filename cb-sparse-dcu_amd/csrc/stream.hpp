@@ -61,6 +61,7 @@ struct BatchPlan {
     A* y;                     // column-major m x nrhs
     uint64_t ldx, ldy;
     A* xg;                    // nrhs > 1: x interleaved in groups of 4 right-hand sides, 4 n per group
+    uint32_t xg_block0;       // the plan's first block of the interleave (batch_prologue)
     A* partial;               // nrhs x partial_stride
     uint32_t* counters;       // nrhs x counter_stride (zero between launches)
     uint64_t partial_stride, counter_stride;
@@ -72,15 +73,20 @@ struct XcdSplit {
     uint32_t begin[kXcds + 1];
 };
 
+// Interleaved columns (groups * n) per block of batch_prologue.
+constexpr uint32_t kPrologueCols = 1024;
+
 // Runs before every launch_batch, on the same stream: zeroes the kXcds unit
 // queue counters and fills BatchPlan::xg of every plan with nrhs > 1 from
-// its x (max_cols: the largest groups * n over those plans, 0 if none).
-hipError_t launch_prologue(const BatchPlan<double, double>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
-                           hipStream_t s);
-hipError_t launch_prologue(const BatchPlan<float, float>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
-                           hipStream_t s);
-hipError_t launch_prologue(const BatchPlan<dev::bf16_t, float>* p, uint32_t np, uint32_t max_cols, uint32_t* queue,
-                           hipStream_t s);
+// its x.  block_plan[b] (device memory, nblocks entries, 0 if no plan has
+// several right-hand sides) is the plan of block b; plan p's blocks start
+// at p.xg_block0 and cover its groups * n columns kPrologueCols at a time.
+hipError_t launch_prologue(const BatchPlan<double, double>* p, const uint32_t* block_plan, uint32_t nblocks,
+                           uint32_t* queue, hipStream_t s);
+hipError_t launch_prologue(const BatchPlan<float, float>* p, const uint32_t* block_plan, uint32_t nblocks,
+                           uint32_t* queue, hipStream_t s);
+hipError_t launch_prologue(const BatchPlan<dev::bf16_t, float>* p, const uint32_t* block_plan, uint32_t nblocks,
+                           uint32_t* queue, hipStream_t s);
 
 // Resident workgroups per CU of the batched kernel for this shape (0 = unknown).
 int batch_blocks_per_cu(int dtype, bool comp, int threads, size_t lds);
