@@ -213,9 +213,14 @@ typedef struct cbspmv_batch cbspmv_batch;
 int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** batch, void* stream);
 /* Y_i = A_i X_i for every matrix i, with nrhs[i] (0..16) right-hand sides
  * each: d_x[i] is n_i x nrhs[i] (column-major, leading dim n_i), d_y[i] is
- * m_i x nrhs[i], of the plans' vector type.  One persistent launch.  The
- * pointer table is uploaded (with a stream synchronisation) only when the
- * pointers or counts differ from the previous call. */
+ * m_i x nrhs[i], of the plans' vector type.  Two launches on `stream`: a
+ * prologue (x of the plans with several right-hand sides interleaved in
+ * groups of four into a batch-owned buffer, work queues reset) and the
+ * persistent batched kernel.  The pointer table is uploaded (with a stream
+ * synchronisation) only when the pointers or counts differ from the
+ * previous call.  Calls, captures and replays of one batch share its
+ * scratch buffers: they must not overlap in time (one stream, or
+ * synchronise between streams). */
 int cbspmv_batch_spmv(cbspmv_batch* batch, const void* const* d_x, void* const* d_y,
                       const int* nrhs, void* stream);
 /* Captures that launch for these pointers into a hipGraph (stream must be a
