@@ -83,17 +83,16 @@ void fill_args(const cbspmv_plan& p, StreamArgs<V, A>& a) {
     a.n = h.n;
 }
 
-constexpr uint32_t kRhsGroup = 4;   // right-hand sides per interleaved group (stream.hip kMultiRhs)
 
 // Estimated cost of one work unit, in stream entries: a tile with one
 // right-hand side streams its entries once with its LDS x cache; with
-// several, once per group of kRhsGroup, gathering every entry's x values
+// several, once per group of (uint32_t)kRhsGroup, gathering every entry's x values
 // (about 1.5x the one-RHS cost per entry); plus a fixed cost per unit.
 double unit_cost(uint32_t entries, int nrhs) {
     constexpr double kUnitFixed = 2048.0, kMultiWeight = 1.5;
     if (nrhs <= 0) return kUnitFixed;
     if (nrhs == 1) return kUnitFixed + entries;
-    return kUnitFixed + kMultiWeight * entries * (double)(((uint32_t)nrhs + kRhsGroup - 1) / kRhsGroup);
+    return kUnitFixed + kMultiWeight * entries * (double)(((uint32_t)nrhs + (uint32_t)kRhsGroup - 1) / (uint32_t)kRhsGroup);
 }
 
 // Contiguous unit ranges of (about) equal estimated cost, one per XCD.
@@ -133,8 +132,8 @@ int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* 
         if (nrhs[i] < 0 || nrhs[i] > kBatchMaxRhs) return fail(CBSPMV_ERR_ARG, "nrhs must be in 0..16");
         if (nrhs[i] > 0 && ((h.m > 0 && !y[i]) || (h.nnz > 0 && !x[i]))) return fail(CBSPMV_ERR_ARG, "null vector");
         if (nrhs[i] > 1) {
-            const uint64_t cols = ((uint64_t)nrhs[i] + kRhsGroup - 1) / kRhsGroup * (uint64_t)h.n;
-            xg_elems += (size_t)cols * kRhsGroup;
+            const uint64_t cols = ((uint64_t)nrhs[i] + (uint32_t)kRhsGroup - 1) / (uint32_t)kRhsGroup * (uint64_t)h.n;
+            xg_elems += (size_t)cols * (uint32_t)kRhsGroup;
             cols_sum += cols;
         }
     }
@@ -170,9 +169,9 @@ int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* 
         e.xg = nullptr;
         e.xg_block0 = (uint32_t)block_plan.size();
         if (nrhs[i] > 1) {
-            const size_t cols = (size_t)((uint32_t)nrhs[i] + kRhsGroup - 1) / kRhsGroup * (size_t)h.n;
+            const size_t cols = (size_t)((uint32_t)nrhs[i] + (uint32_t)kRhsGroup - 1) / (uint32_t)kRhsGroup * (size_t)h.n;
             e.xg = static_cast<A*>(xg.ptr) + xg_off;
-            xg_off += cols * kRhsGroup;
+            xg_off += cols * (uint32_t)kRhsGroup;
             block_plan.insert(block_plan.end(), (cols + kPrologueCols - 1) / kPrologueCols, (uint32_t)i);
         }
         part += (size_t)h.partial_slots * parts * kBatchMaxRhs;

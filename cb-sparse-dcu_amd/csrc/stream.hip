@@ -32,7 +32,7 @@
 namespace cbs {
 namespace dev {
 
-constexpr int kMultiRhs = 4;   // right-hand sides reduced per chunk load
+constexpr int kMultiRhs = kRhsGroup;   // right-hand sides reduced per chunk load
 
 template <typename A>
 __device__ __forceinline__ A load_partial(const A* p) {
@@ -613,24 +613,29 @@ __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t 
 }
 
 // The R values x[col] of R right-hand sides, from x interleaved by column
-// (p = xg + col * R; 16-byte aligned): one 16-byte load per 4 floats or 2
-// doubles, so the R right-hand sides cost R/4 (FP32) or R/2 (FP64) gathers
-// per entry instead of R.
+// (p = xg + col * R; 16-byte aligned): 16-byte loads issued back to back
+// over one column's R values, which the vector L1 serves as about one
+// gather (a probe of random columns: one 16-byte load 265-290 G/s, two of
+// one 32-byte column 227-238 G/s, two of separate columns 133-142 G/s).
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
+// Only the vectors holding one of the nr (wave-uniform) right-hand sides in
+// use are loaded; the others stay 0.
 template <int R>
-__device__ __forceinline__ void load_rhs(const float* p, float (&v)[R]) {
+__device__ __forceinline__ void load_rhs(const float* p, float (&v)[R], uint32_t nr) {
 #pragma unroll
     for (int q = 0; q < R / 4; ++q) {
-        const f32x4 w = *reinterpret_cast<const f32x4*>(p + 4 * q);
+        f32x4 w = {0.f, 0.f, 0.f, 0.f};
+        if ((uint32_t)(4 * q) < nr) w = *reinterpret_cast<const f32x4*>(p + 4 * q);
         v[4 * q] = w.x; v[4 * q + 1] = w.y; v[4 * q + 2] = w.z; v[4 * q + 3] = w.w;
     }
 }
 template <int R>
-__device__ __forceinline__ void load_rhs(const double* p, double (&v)[R]) {
+__device__ __forceinline__ void load_rhs(const double* p, double (&v)[R], uint32_t nr) {
 #pragma unroll
     for (int q = 0; q < R / 2; ++q) {
-        const f64x2 w = *reinterpret_cast<const f64x2*>(p + 2 * q);
+        f64x2 w = {0.0, 0.0};
+        if ((uint32_t)(2 * q) < nr) w = *reinterpret_cast<const f64x2*>(p + 2 * q);
         v[2 * q] = w.x; v[2 * q + 1] = w.y;
     }
 }
@@ -720,7 +725,7 @@ __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t i
         const bool carried = carry_off >= 0 && o0 == carry_off;
         A xr[EPL][R];
 #pragma unroll
-        for (int j = 0; j < EPL; ++j) load_rhs<R>(a.x + (size_t)col[j] * R, xr[j]);
+        for (int j = 0; j < EPL; ++j) load_rhs<R>(a.x + (size_t)col[j] * R, xr[j], nr);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if ((uint32_t)r >= nr) break;
@@ -910,18 +915,26 @@ __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan
 }
 
 // batch_prologue writes the x of every plan with several right-hand sides
-// interleaved in groups of kMultiRhs for stream_tile_multi: xg[(g*n + col)*kMultiRhs + j] =
-// x[(g*kMultiRhs + j)*ldx + col], 0 past the last right-hand side.  Reads
-// are coalesced over col, each thread writes one 16-byte (FP32) or two
-// 16-byte (FP64) vectors.
-__device__ __forceinline__ void store_rhs(float* p, const float (&v)[4]) {
-    *reinterpret_cast<f32x4*>(p) = f32x4{v[0], v[1], v[2], v[3]};
+// interleaved in groups of kMultiRhs for stream_tile_multi:
+// xg[(g*n + col)*kMultiRhs + j] = x[(g*kMultiRhs + j)*ldx + col].  Reads are
+// coalesced over col; each thread writes the 16-byte vectors of one column
+// that hold a right-hand side (entries past the last one are 0; vectors
+// holding none are neither written nor loaded by load_rhs).
+template <int R>
+__device__ __forceinline__ void store_rhs(float* p, const float (&v)[R], uint32_t nr) {
+    static_assert(R % 4 == 0, "whole 16-byte vectors");
+#pragma unroll
+    for (int q = 0; q < R / 4; ++q)
+        if ((uint32_t)(4 * q) < nr)
+            reinterpret_cast<f32x4*>(p)[q] = f32x4{v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]};
 }
-__device__ __forceinline__ void store_rhs(double* p, const double (&v)[4]) {
-    reinterpret_cast<f64x2*>(p)[0] = f64x2{v[0], v[1]};
-    reinterpret_cast<f64x2*>(p)[1] = f64x2{v[2], v[3]};
+template <int R>
+__device__ __forceinline__ void store_rhs(double* p, const double (&v)[R], uint32_t nr) {
+    static_assert(R % 2 == 0, "whole 16-byte vectors");
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q)
+        if ((uint32_t)(2 * q) < nr) reinterpret_cast<f64x2*>(p)[q] = f64x2{v[2 * q], v[2 * q + 1]};
 }
-static_assert(kMultiRhs == 4, "store_rhs writes groups of 4");
 
 // Runs before every stream_batch launch: zeroes the per-XCD unit queues
 // and interleaves x.  Block b works on plan block_plan[b] (a table the host
@@ -948,7 +961,7 @@ __global__ __launch_bounds__(256) void batch_prologue(const BatchPlan<V, A>* pla
             const uint32_t r = g * kMultiRhs + (uint32_t)j;
             v[j] = r < bp.nrhs ? bp.x[(size_t)r * bp.ldx + col] : A(0);
         }
-        store_rhs(bp.xg + (size_t)c * kMultiRhs, v);
+        store_rhs<kMultiRhs>(bp.xg + (size_t)c * kMultiRhs, v, bp.nrhs - g * kMultiRhs);
     }
 }
 

@@ -11,10 +11,13 @@ namespace cbs {
 namespace dev { struct bf16_t; }
 
 constexpr int kMaxStreamThreads = 64 * kMaxTileItems;   // one wave per tile item
-// The batched kernel has at most 10 waves per workgroup (so each may hold
-// more registers for several right-hand sides); the items of a tile with
-// more are dealt to its waves in turn.
-constexpr int kMaxBatchTileItems = 10;
+// The batched kernel has at most 8 waves per workgroup (each holds the x
+// values of eight right-hand sides for a chunk: ~210 VGPRs); the items of a
+// tile with more are dealt to its waves in turn.
+constexpr int kMaxBatchTileItems = 8;
+// Right-hand sides of one pass of the batched kernel over a tile, and the
+// width of the groups x is interleaved in for it (BatchPlan::xg).
+constexpr int kRhsGroup = 8;
 constexpr int kMaxBatchThreads = 64 * kMaxBatchTileItems;
 
 template <typename V, typename A>
@@ -60,7 +63,7 @@ struct BatchPlan {
     const A* x;               // column-major n x nrhs
     A* y;                     // column-major m x nrhs
     uint64_t ldx, ldy;
-    A* xg;                    // nrhs > 1: x interleaved in groups of 4 right-hand sides, 4 n per group
+    A* xg;                    // nrhs > 1: x interleaved in groups of kRhsGroup right-hand sides
     uint32_t xg_block0;       // the plan's first block of the interleave (batch_prologue)
     A* partial;               // nrhs x partial_stride
     uint32_t* counters;       // nrhs x counter_stride (zero between launches)

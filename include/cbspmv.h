@@ -215,7 +215,7 @@ int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** bat
  * each: d_x[i] is n_i x nrhs[i] (column-major, leading dim n_i), d_y[i] is
  * m_i x nrhs[i], of the plans' vector type.  Two launches on `stream`: a
  * prologue (x of the plans with several right-hand sides interleaved in
- * groups of four into a batch-owned buffer, work queues reset) and the
+ * groups of eight into a batch-owned buffer, work queues reset) and the
  * persistent batched kernel.  The pointer table is uploaded (with a stream
  * synchronisation) only when the pointers or counts differ from the
  * previous call.  Calls, captures and replays of one batch share its

@@ -70,10 +70,12 @@ def test_batch_parity(gpu, dtype, comp):
     tdt = torch.float64 if dtype == "f64" else torch.float32
     rng = np.random.default_rng(1)
     plans, X, Y = [], [], []
-    # tiles of more than 10 items (12, 16) are dealt to the batch kernel's
-    # 10 waves in turn, on both the one- and the several-right-hand-side path
+    # tiles of more than 8 items (9, 12, 16) are dealt to the batch kernel's
+    # 8 waves in turn, on both the one- and the several-right-hand-side path;
+    # 3, 5, 6, 8 and 16 right-hand sides fill one group of eight partly (one
+    # or two 16-byte vectors in FP32), exactly, and twice
     waves = [4, 9, 16, 12, 8, 16, 4, 12, 9, 12, 16]
-    rhs = [1, 3, 1, 16, 8, 8, 16, 1, 3, 16, 1]
+    rhs = [1, 3, 1, 16, 8, 5, 16, 1, 6, 16, 1]
     for i, (m, n, r, c, v) in enumerate(mats):
         opts = dict(dtype=dtype, compensated=comp, tile_waves=waves[i % 11], item_nnz=[0, 256, 0][i % 3])
         plans.append(Plan(m, n, r, c, v, **opts))
