@@ -858,23 +858,29 @@ __global__ __launch_bounds__(256) void panel_sums(A* y, const A* out, const uint
 // owns the contiguous unit range split.begin[xcd] .. split.begin[xcd + 1]
 // (ranges of equal estimated cost, so a plan's tiles share one L2), and its
 // workgroups take the next unit of that range from queue[xcd] (zeroed by
-// batch_prologue before the launch).  A unit runs its tile for every
-// right-hand side of its plan in turn.  Unit 0 of a plan also zeroes the
-// plan's empty rows.
+// batch_prologue before the launch).  A workgroup whose XCD's range is done
+// goes on with the other XCDs' queues in turn, so an error of the cost
+// estimate leaves no XCD idle; a unit is taken once whoever takes it.  A
+// unit runs its tile for every right-hand side of its plan in turn.  Unit 0
+// of a plan also zeroes the plan's empty rows.
 template <typename V, typename A, bool COMP, int EPL>
 __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan<V, A>* plans, uint32_t nplans,
                                                                   XcdSplit split, uint32_t* queue) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ uint32_t next_unit;
     A* xs = reinterpret_cast<A*>(lds_raw);
-    const uint32_t xcd = blockIdx.x & 7u;   // the hardware deals workgroups round-robin over the 8 XCDs
-    const uint32_t u1 = split.begin[xcd + 1];
+    uint32_t q = blockIdx.x & 7u;   // the hardware deals workgroups round-robin over the 8 XCDs
+    uint32_t queues_done = 0;
     for (;;) {
-        if (threadIdx.x == 0) next_unit = split.begin[xcd] + atomicAdd(queue + xcd, 1u);
+        if (threadIdx.x == 0) next_unit = split.begin[q] + atomicAdd(queue + q, 1u);
         __syncthreads();
         const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)next_unit);
         __syncthreads();   // next_unit is rewritten at the next unit
-        if (u >= u1) break;
+        if (u >= split.begin[q + 1]) {
+            if (++queues_done == (uint32_t)kXcds) break;
+            q = (q + 1u) & 7u;
+            continue;
+        }
         // plan of unit u: last plan with unit_begin <= u (binary search)
         uint32_t lo = 0, hi = nplans;
         while (hi - lo > 1) {
