@@ -86,7 +86,7 @@ void fill_args(const cbspmv_plan& p, StreamArgs<V, A>& a) {
 
 // Estimated cost of one work unit, in stream entries: a tile with one
 // right-hand side streams its entries once with its LDS x cache; with
-// several, once per group of (uint32_t)kRhsGroup, gathering every entry's x values
+// several, once per group of kRhsGroup, gathering every entry's x values
 // (about 1.5x the one-RHS cost per entry); plus a fixed cost per unit.
 double unit_cost(uint32_t entries, int nrhs) {
     constexpr double kUnitFixed = 2048.0, kMultiWeight = 1.5;
@@ -138,7 +138,7 @@ int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* 
         }
     }
     if (cols_sum >= (1ull << 31))
-        return fail(CBSPMV_ERR_UNSUPPORTED, "batch: sum over plans of n x ceil(nrhs / 4) must stay below 2^31");
+        return fail(CBSPMV_ERR_UNSUPPORTED, "batch: sum over plans of n x ceil(nrhs / 8) must stay below 2^31");
     std::vector<uint32_t> block_plan;
     if (xg.bytes < xg_elems * sizeof(A)) {
         const int rc = upload_bytes(xg, nullptr, xg_elems * sizeof(A), s);
