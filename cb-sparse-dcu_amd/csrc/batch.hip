@@ -83,7 +83,6 @@ void fill_args(const cbspmv_plan& p, StreamArgs<V, A>& a) {
     a.n = h.n;
 }
 
-
 // Estimated cost of one work unit, in stream entries: a tile with one
 // right-hand side streams its entries once with its LDS x cache; with
 // several, once per group of kRhsGroup, gathering every entry's x values
