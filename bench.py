@@ -466,7 +466,7 @@ def main():
                     help="only run a plain 256 MiB device-to-device copy (for scaling PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU runs are out of scope", file=sys.stderr)
         sys.exit(2)
 
@@ -541,7 +541,7 @@ def main():
     workload = (f"config{args.config}_{args.dtype}{'_comp' if args.compensated else ''}_bs{args.block_size}")
     if not no_cpu:
         cpu_gbs = b_alg / t_cpu / 1e9
-        threads = min(16, os.cpu_count() or 1)   # the host cores one GPU's share of the box has
+        threads = min(16, os.cpu_count() or 1)   # the host cores that go with one GPU
         t_omp = cpu_parallel(d, threads)
         cpu = {"value": round(cpu_gbs, 3), "unit": "GB/s", "cores": 1, "kind": kind,
                "sample": f"full workload ({nnz} nnz), median of {reps} runs of {what}; "

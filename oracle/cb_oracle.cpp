@@ -1,5 +1,5 @@
 // CB-SpMV oracle: a plain, single-threaded CPU restatement of the reference
-// algorithm, used ONLY as the checker by tests/, __graft_entry__.smoke() and
+// algorithm, used ONLY as the oracle by tests/, __graft_entry__.smoke() and
 // the cpu_baseline leg of bench.py.  Nothing in the product links or loads it.
 //
 // What it restates (reference paths are relative to the reference checkout):

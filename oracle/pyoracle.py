@@ -1,4 +1,4 @@
-"""Python access to the checkers (test infrastructure only).
+"""Python access to the oracles (test infrastructure only).
 
   liboracle.so      CPU restatement of the reference path (cb_oracle.cpp)
   _ref/ref_*.so     the reference's own host code compiled from its sources
@@ -6,7 +6,7 @@
                     checkout is present)
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
-module, and only as the checker.
+module, and only as the oracle.
 """
 import ctypes
 import os

@@ -1,4 +1,4 @@
-// Checker-side shim around the reference's own host code, compiled from the
+// Oracle-side shim around the reference's own host code, compiled from the
 // reference sources where they lie (see oracle/Makefile; nothing is copied).
 // The resulting oracle/_ref/*.so is test infrastructure: tests compare the
 // oracle's restatement and the product builder against it, and bench.py's
