@@ -309,12 +309,29 @@ void layout_tiles(Plan& p, const StreamShape& sh, const Entries& E, std::vector<
     if (p.o.item_nnz > 0) {
         cut(std::max<int64_t>(64, std::min<int64_t>(p.o.item_nnz, 1 << 22)));
     } else {
+        // the smallest item size whose cut fits one round of tiles (one per
+        // CU): cutting at row boundaries adds items, so grow the target until
+        // the cut fits, then bisect back towards the smallest that fits, so
+        // that few CUs are left without a tile (30M entries in FP64: 256
+        // tiles instead of 249)
         const int64_t max_items = (int64_t)kStreamCUs * p.o.tile_items;
-        int64_t target = std::max<int64_t>(1024, total / max_items + 1);
-        for (int round = 0; round < 64; ++round) {
-            cut(target);
-            if ((int64_t)pieces.size() <= max_items || target >= (1 << 22)) break;
-            target = target + target / 16 + 1;
+        int64_t lo = std::max<int64_t>(1024, total / max_items + 1), hi = lo;
+        cut(hi);
+        bool fits = (int64_t)pieces.size() <= max_items;
+        for (int round = 0; round < 64 && !fits && hi < (1 << 22); ++round) {
+            lo = hi + 1;
+            hi = hi + hi / 16 + 1;
+            cut(hi);
+            fits = (int64_t)pieces.size() <= max_items;
+        }
+        if (fits && lo < hi) {
+            while (lo < hi) {   // hi fits, lo - 1 did not
+                const int64_t mid = lo + (hi - lo) / 2;
+                cut(mid);
+                if ((int64_t)pieces.size() <= max_items) hi = mid;
+                else lo = mid + 1;
+            }
+            cut(hi);
         }
     }
     append_items(p, E, pieces, sh.epl, p_col, p_off, p_zero, &src);

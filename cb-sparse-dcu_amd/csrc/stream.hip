@@ -814,11 +814,14 @@ template <typename V, typename A, bool COMP, int EPL, bool SDDMM, bool NEAR>
 __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     A* xs = reinterpret_cast<A*>(lds_raw);   // tile_slots elements (dynamic LDS)
-    // XCD-aware order: the hardware deals workgroups round-robin over the 8
-    // XCDs, so map them to contiguous tile ranges per XCD (x windows of
-    // neighbouring tiles then share that XCD's L2).
-    uint32_t tile = blockIdx.x;
-    if ((gridDim.x & 7u) == 0) tile = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    // Workgroup b streams tile b.  The hardware deals workgroups round-robin
+    // over the 8 XCDs, so neighbouring tiles (rows) land on different XCDs
+    // and a matrix whose costly rows are contiguous (a 1M-row matrix with a
+    // stencil half and a power-law half with scattered columns) spreads its
+    // gathers over every XCD's L2; contiguous tile ranges per XCD (for
+    // shared x windows) were 3.6% slower there and no faster on banded
+    // matrices.
+    const uint32_t tile = blockIdx.x;
     const uint32_t nthreads = blockDim.x;
     for (uint32_t t = blockIdx.x * nthreads + threadIdx.x; t < a.n_empty_rows; t += gridDim.x * nthreads)
         a.y[a.empty_rows[t]] = A(0);
