@@ -180,3 +180,19 @@ def test_default_tile_waves():
         plan.build_transpose()
         assert int(plan.export("transpose.stream_tile_shape")[0]) == want_t, kw
         plan.close()
+
+
+def test_items_fill_one_round_of_tiles():
+    """Items are cut as small as one round of tiles (one per CU, 256) allows:
+    never more items than 256 tiles hold, and for a matrix of many short
+    rows hardly fewer, so few CUs are left without a tile.  (The matrix is
+    large enough that items stay above their 1024-entry minimum.)"""
+    m = n = 400_000
+    r, c, v = gen.random_coo(m, n, 12, seed=31)
+    for dtype in ("f64", "f32"):
+        plan = Plan(m, n, r, c, v, dtype=dtype)
+        waves = tile_shape(plan)[0]
+        items = plan.export("stream_items").reshape(-1, 4).shape[0]
+        assert items <= 256 * waves, dtype
+        assert items >= 0.99 * 256 * waves, (dtype, items)
+        plan.close()
