@@ -59,7 +59,7 @@ int build_plan_object(const cbspmv_opts& o, int m, int n, int nnz, const int* ro
     h.o.item_nnz = o.stream_item_nnz;
     h.o.panels = o.stream_panels;
     if (o.stream_tile_waves) h.o.tile_items = o.stream_tile_waves;
-    else if (o.value_dtype != CBSPMV_F64 && h.o.panels <= 1) h.o.tile_items = kWideTileItemsOpt;
+    else if (h.o.panels <= 1) h.o.tile_items = kWideTileItemsOpt;
     if (o.stream_tile_kib) h.o.tile_bytes = o.stream_tile_kib * 1024;
     if (h.o.panels > 1 && h.o.tile_items > kMaxTileItems / 2) {
         delete p;

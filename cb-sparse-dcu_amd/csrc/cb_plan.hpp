@@ -16,9 +16,9 @@ namespace cbs {
 // Block type codes and flags (cb-spmv-dcu/src/macros.h:99-105).
 enum : uint8_t { kCOO = 0, kCSR = 1, kDENSE = 3, kTypeMask = 0x7f, kUniform = 0x80 };
 
-// Waves per stream tile when the options leave it open: 8, or 16 for 4-byte
-// values without column panels (their forward launch hides more gather
-// latency with 16 waves per CU; 8-byte values gain nothing from it).
+// Waves per stream tile when the options leave it open: 16 without column
+// panels (the forward launch hides more gather latency with 16 waves per
+// CU), else 8 (a workgroup adds as many panel waves).
 constexpr int kDefaultTileItemsOpt = 8;
 constexpr int kWideTileItemsOpt = 16;
 constexpr int kDefaultTileBytesOpt = 96 * 1024;

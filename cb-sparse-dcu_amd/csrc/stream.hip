@@ -182,8 +182,8 @@ __device__ __forceinline__ f2 wave_shr1(f2 v) { return f2{dpp<kWaveShr1, 0xf>(v.
 // Every thread of the workgroup calls this; it ends with a workgroup barrier.
 template <typename V, typename A>
 __device__ __forceinline__ void stage_tile(const StreamArgs<V, A>& a, uint32_t tile, A* xs) {
-    // one pass for the default 96 KB cache and tile shapes: 12,288 FP64
-    // slots over 8 waves, 24,576 FP32 slots over 16
+    // one pass for the default 96 KB cache and 16 waves: up to 24,576
+    // (FP32) slots over 1024 threads
     constexpr int PER = 24;
     const uint32_t nthreads = blockDim.x;
     const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1] * kLineElems;

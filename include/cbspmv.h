@@ -75,8 +75,8 @@ typedef struct cbspmv_opts {
     int schedule;         /* enum cbspmv_schedule                                    */
     int stream_item_nnz;  /* stream schedule: target entries per wave item (0=auto)  */
     int stream_tile_waves;/* stream schedule: waves per workgroup, 1..16 (0 = auto:
-                             16 for FP32/BF16 values without column panels, else 8;
-                             8 for the transposed plan of the backward pass)          */
+                             16 without column panels, else 8; 8 for the
+                             transposed plan of the backward pass)                    */
     int stream_tile_kib;  /* stream schedule: LDS x cache per workgroup, 4..156 KiB (0=96) */
     int backward_mode;    /* enum cbspmv_backward_mode (cbspmv_backward)              */
     int stream_panels;    /* stream schedule: x column panels 1, 2, 4 or 8 (0 = 1)    */

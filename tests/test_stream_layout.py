@@ -167,12 +167,12 @@ def test_stream_spmv_emulated_matches_oracle():
 
 
 def test_default_tile_waves():
-    """Waves per tile left open: 16 for 4-byte values without column panels,
-    8 for FP64, for panel plans and for the transposed plan of a backward
-    pass; an explicit count is kept everywhere."""
+    """Waves per tile left open: 16 without column panels, 8 for panel plans
+    and for the transposed plan of a backward pass; an explicit count is
+    kept everywhere."""
     m, n = 500, 700
     r, c, v = gen.random_coo(m, n, 6, seed=21)
-    for kw, want, want_t in ((dict(dtype="f64"), 8, 8), (dict(dtype="f32"), 16, 8), (dict(dtype="bf16"), 16, 8),
+    for kw, want, want_t in ((dict(dtype="f64"), 16, 8), (dict(dtype="f32"), 16, 8), (dict(dtype="bf16"), 16, 8),
                              (dict(dtype="f32", compensated=True), 16, 8), (dict(dtype="f32", panels=2), 8, 8),
                              (dict(dtype="f32", tile_waves=5), 5, 5), (dict(dtype="f64", tile_waves=12), 12, 12)):
         plan = Plan(m, n, r, c, v, **kw)
