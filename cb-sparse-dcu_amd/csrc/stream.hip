@@ -909,7 +909,7 @@ __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan
             continue;
         }
         // several: groups of up to kMultiRhs share each chunk load and each
-        // x gather (x interleaved per group by interleave_rhs)
+        // x gather (x interleaved per group by batch_prologue)
         for (uint32_t r0 = 0; r0 < bp.nrhs; r0 += kMultiRhs) {
             StreamArgs<V, A> a = bp.args;
             a.x = bp.xg + (size_t)r0 * bp.args.n;
