@@ -643,7 +643,7 @@ __device__ __forceinline__ void load_rhs(const double* p, double (&v)[R], uint32
 // Several right-hand sides at once: each chunk of keys and values is loaded
 // once and reduced for every right-hand side r < nr (at most R), so the slab
 // is read once per tile instead of once per right-hand side.  x comes
-// interleaved (a.x[col * R + r], written by interleave_rhs) and is gathered
+// interleaved (a.x[col * R + r], written by batch_prologue) and is gathered
 // from global memory (L2) for every entry, all R values of a column with
 // one vector load; LDS holds the tile's line table to decode the column of
 // an entry whose key names an LDS slot.  RHS r writes y + r*ldy; pieces of
