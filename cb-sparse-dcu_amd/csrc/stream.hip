@@ -644,8 +644,9 @@ __device__ __forceinline__ void load_rhs(const double* p, double (&v)[R], uint32
 // once and reduced for every right-hand side r < nr (at most R), so the slab
 // is read once per tile instead of once per right-hand side.  x comes
 // interleaved (a.x[col * R + r], written by batch_prologue) and is gathered
-// from global memory (L2) for every entry, all R values of a column with
-// one vector load; LDS holds the tile's line table to decode the column of
+// from global memory (L2) for every entry, the R values of a column with
+// back-to-back 16-byte loads (load_rhs); LDS holds the tile's line table to
+// decode the column of
 // an entry whose key names an LDS slot.  RHS r writes y + r*ldy; pieces of
 // cut rows go to partial + r*pstride and share one ticket per split.
 // multi_item runs one item with one wave; stream_tile_multi deals the tile's
