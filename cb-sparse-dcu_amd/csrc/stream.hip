@@ -646,11 +646,10 @@ __device__ __forceinline__ void load_rhs(const double* p, double (&v)[R], uint32
 // interleaved (a.x[col * R + r], written by batch_prologue) and is gathered
 // from global memory (L2) for every entry, the R values of a column with
 // back-to-back 16-byte loads (load_rhs); LDS holds the tile's line table to
-// decode the column of
-// an entry whose key names an LDS slot.  RHS r writes y + r*ldy; pieces of
-// cut rows go to partial + r*pstride and share one ticket per split.
-// multi_item runs one item with one wave; stream_tile_multi deals the tile's
-// items to the workgroup's waves.
+// decode the column of an entry whose key names an LDS slot.  RHS r writes
+// y + r*ldy; pieces of cut rows go to partial + r*pstride and share one
+// ticket per split.  multi_item runs one item with one wave;
+// stream_tile_multi deals the tile's items to the workgroup's waves.
 template <typename V, typename A, bool COMP, int EPL, int R>
 __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t item, const uint32_t* lines,
                                            uint64_t ldy, uint64_t pstride, uint32_t nr) {
