@@ -63,13 +63,23 @@ class _SpMV(torch.autograd.Function):
         x = x.contiguous()
         y = mat.plan.spmv(x)
         ctx.mat = mat
-        ctx.save_for_backward(x)
+        ctx.has_vals = vals is not None
+        # saving vals lets autograd's version check reject an in-place update
+        # between forward and backward; backward pushes them again in case
+        # another forward pushed other values to the same matrix meanwhile
+        if vals is not None:
+            ctx.save_for_backward(x, vals)
+        else:
+            ctx.save_for_backward(x)
         return y
 
     @staticmethod
     def backward(ctx, g):
-        (x,) = ctx.saved_tensors
+        saved = ctx.saved_tensors
+        x = saved[0]
         want_vals, want_x = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        if ctx.has_vals and want_x:
+            ctx.mat.sync_values(saved[1])   # dx = A(vals)^T g needs these values on the device
         dx, dvals = ctx.mat.plan.backward(g.contiguous(), x, want_dx=want_x, want_dvals=want_vals)
         return dvals, dx, None
 

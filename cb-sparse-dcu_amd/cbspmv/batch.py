@@ -67,9 +67,12 @@ class Batch:
         return Y
 
     def capture(self, X, Y, stream):
-        """Captures the launch into a graph (on a created stream) and runs it."""
+        """Captures the launch into a graph (on a created stream) and runs it.
+        The graph holds the tensors' device addresses, so the batch keeps
+        X and Y alive until the next capture or close()."""
         xs, ys, nr = self._tables(X, Y)
         check(lib().cbspmv_batch_capture(self._h, xs, ys, nr, _stream_handle(stream)))
+        self._captured = (list(X), list(Y))
 
     def replay(self, stream):
         check(lib().cbspmv_batch_replay(self._h, _stream_handle(stream)))
@@ -78,6 +81,7 @@ class Batch:
         if getattr(self, "_h", None) and self._h.value:
             lib().cbspmv_batch_destroy(self._h)
             self._h = ctypes.c_void_p()
+        self._captured = None
 
     def __del__(self):
         try:

@@ -27,8 +27,13 @@
 //     running on it.  The reference packer itself allocates device memory
 //     before it computes addresses, so it cannot run here: the byte layout is
 //     "parity pinned by invariants", not by a reference image.
+#include <omp.h>
+#include <sched.h>
+
 #include <algorithm>
+#include <chrono>
 #include <climits>
+#include <thread>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -747,6 +752,34 @@ void oracle_spmv_csr_omp(int m, const long* row_ptr, const int* col, const doubl
         for (long k = row_ptr[r]; k < row_ptr[r + 1]; ++k) s += val[k] * x[col[k]];
         y[r] = s;
     }
+}
+
+// Median seconds per call of oracle_spmv_csr_omp over `reps` calls after one
+// warm-up call; the caller builds the CSR arrays once, so only the SpMV
+// loop is timed.
+double oracle_time_csr_omp(int m, const long* row_ptr, const int* col, const double* val,
+                           const double* x, double* y, int threads, int reps) {
+    oracle_spmv_csr_omp(m, row_ptr, col, val, x, y, threads);
+    std::vector<double> t;
+    for (int k = 0; k < reps; ++k) {
+        const auto a = std::chrono::steady_clock::now();
+        oracle_spmv_csr_omp(m, row_ptr, col, val, x, y, threads);
+        const auto b = std::chrono::steady_clock::now();
+        t.push_back(std::chrono::duration<double>(b - a).count());
+    }
+    std::sort(t.begin(), t.end());
+    return t.empty() ? 0.0 : t[t.size() / 2];
+}
+
+// The host's thread counts: omp_get_max_threads(),
+// std::thread::hardware_concurrency() and the size of this process's CPU
+// affinity mask.
+void oracle_host_threads(int* omp_max, int* hw, int* affinity) {
+    *omp_max = omp_get_max_threads();
+    *hw = (int)std::thread::hardware_concurrency();
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    *affinity = sched_getaffinity(0, sizeof(set), &set) == 0 ? CPU_COUNT(&set) : -1;
 }
 
 }  // extern "C"

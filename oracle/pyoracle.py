@@ -61,6 +61,10 @@ def oracle_lib():
         L.oracle_policy.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
         L.oracle_spmv_seq.argtypes = [ctypes.c_int, ip, ip, dp, ctypes.c_int, dp, dp]
         L.oracle_spmv_csr_omp.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_long), ip, dp, dp, dp, ctypes.c_int]
+        L.oracle_time_csr_omp.restype = ctypes.c_double
+        L.oracle_time_csr_omp.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_long), ip, dp, dp, dp, ctypes.c_int,
+                                          ctypes.c_int]
+        L.oracle_host_threads.argtypes = [ip, ip, ip]
         _oracle = L
     return _oracle
 
@@ -189,16 +193,51 @@ def spmv_seq(m, row, col, val, x):
     return y
 
 
+def csr_row_ptr(m, row):
+    """int64 row pointer of a row-major COO row array (rows non-decreasing)."""
+    ptr = np.zeros(int(m) + 1, dtype=np.int64)
+    ptr[1:] = np.cumsum(np.bincount(np.asarray(row, dtype=np.int64), minlength=int(m)))
+    return ptr
+
+
+class CSR:
+    """The CSR form of a row-major COO matrix, built once, for repeated
+    row-parallel SpMVs (same per-row order as utils.h:98-111)."""
+
+    def __init__(self, m, row, col, val):
+        self.m = int(m)
+        self.ptr = csr_row_ptr(m, row)
+        self.col, self.val = _i32(col), _f64(val)
+
+    def spmv(self, x, threads=16):
+        x = _f64(x)
+        y = np.zeros(self.m, dtype=np.float64)
+        oracle_lib().oracle_spmv_csr_omp(self.m, _p(self.ptr, ctypes.c_long), _p(self.col, ctypes.c_int),
+                                         _p(self.val, ctypes.c_double), _p(x, ctypes.c_double),
+                                         _p(y, ctypes.c_double), int(threads))
+        return y
+
+
+    def time_spmv(self, x, threads, reps):
+        """Median seconds per SpMV over `reps` runs after one warm-up, timed
+        in C++ around the OpenMP loop alone."""
+        x = _f64(x)
+        y = np.zeros(self.m, dtype=np.float64)
+        return oracle_lib().oracle_time_csr_omp(self.m, _p(self.ptr, ctypes.c_long), _p(self.col, ctypes.c_int),
+                                                _p(self.val, ctypes.c_double), _p(x, ctypes.c_double),
+                                                _p(y, ctypes.c_double), int(threads), int(reps))
+
+
+def host_threads():
+    """{omp_max_threads, hardware_concurrency, affinity_cpus} of this process."""
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    oracle_lib().oracle_host_threads(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return {"omp_max_threads": a.value, "hardware_concurrency": b.value, "affinity_cpus": c.value}
+
+
 def spmv_csr_omp(m, row, col, val, x, threads):
     """Row-parallel CSR form of the same sum (same per-row order)."""
-    row, col, val, x = _i32(row), _i32(col), _f64(val), _f64(x)
-    ptr = np.zeros(int(m) + 1, dtype=np.int64)
-    np.add.at(ptr, row + 1, 1)
-    ptr = np.cumsum(ptr)
-    y = np.zeros(int(m), dtype=np.float64)
-    oracle_lib().oracle_spmv_csr_omp(int(m), _p(ptr, ctypes.c_long), _p(col, ctypes.c_int), _p(val, ctypes.c_double),
-                                     _p(x, ctypes.c_double), _p(y, ctypes.c_double), int(threads))
-    return y
+    return CSR(m, row, col, val).spmv(x, threads)
 
 
 def ref_check(m, n, row, col, val, x, y):

@@ -218,3 +218,43 @@ def test_fp32_gradients(gpu, compensated):
     assert pyoracle.rel_l2(xt.grad.cpu().numpy().astype(np.float64), dx_ref) <= 1e-6
     p.close()
     A.close()
+
+
+@pytest.mark.gpu
+def test_two_value_sets_share_one_matrix(gpu):
+    """Two forwards with different values on one CBSparse before a single
+    backward: each node's x.grad must use its own values (A(v1)^T g1 +
+    A(v2)^T g2), checked against torch.sparse on the CPU in FP64."""
+    import torch
+
+    r, c, v = _matrix(seed=15)
+    A = CBSparse(M, N, r, c, v)
+    rng = np.random.default_rng(16)
+    v1, v2 = rng.uniform(-1, 1, r.size), rng.uniform(-2, 2, r.size)
+    x0 = rng.uniform(-1, 1, N)
+    x = _t(x0, gpu).requires_grad_(True)
+    t1, t2 = _t(v1, gpu).requires_grad_(True), _t(v2, gpu).requires_grad_(True)
+    y1 = cb_spmv_autograd(A, x, t1)
+    y2 = cb_spmv_autograd(A, x, t2)
+    w = torch.arange(M, device=gpu, dtype=torch.float64)
+    (torch.sin(y1).sum() + (torch.cos(y2) * w).sum()).backward()
+
+    tx = torch.tensor(x0, dtype=torch.float64, requires_grad=True)
+    idx = torch.tensor(np.stack([r, c]), dtype=torch.long)
+    tv1 = torch.tensor(v1, dtype=torch.float64, requires_grad=True)
+    tv2 = torch.tensor(v2, dtype=torch.float64, requires_grad=True)
+    ty1 = torch.sparse.mm(torch.sparse_coo_tensor(idx, tv1, (M, N)), tx.view(-1, 1)).view(-1)
+    ty2 = torch.sparse.mm(torch.sparse_coo_tensor(idx, tv2, (M, N)), tx.view(-1, 1)).view(-1)
+    (torch.sin(ty1).sum() + (torch.cos(ty2) * torch.arange(M, dtype=torch.float64)).sum()).backward()
+    assert pyoracle.rel_l2(y1.detach().cpu().numpy(), ty1.detach().numpy()) <= 1e-12
+    assert pyoracle.rel_l2(y2.detach().cpu().numpy(), ty2.detach().numpy()) <= 1e-12
+    assert pyoracle.rel_l2(x.grad.cpu().numpy(), tx.grad.numpy()) <= 1e-12
+    assert pyoracle.rel_l2(t1.grad.cpu().numpy(), tv1.grad.numpy()) <= 1e-12
+    assert pyoracle.rel_l2(t2.grad.cpu().numpy(), tv2.grad.numpy()) <= 1e-12
+    # an in-place update of saved values between forward and backward is refused
+    y3 = cb_spmv_autograd(A, x, t1)
+    with torch.no_grad():
+        t1.add_(1.0)
+    with pytest.raises(RuntimeError):
+        y3.sum().backward()
+    A.close()

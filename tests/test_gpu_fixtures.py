@@ -80,3 +80,56 @@ def test_bench_on_fixture(gpu):
     assert pyoracle.rel_l2(y, pyoracle.spmv_seq(m, r, c, v, x)) <= TOL
     assert runtime > 0
     plan.close()
+
+
+def repeated_patterns(m=6000, n=5000, patterns=200, seed=3):
+    """Unit values, every row a copy of one of `patterns` column sets of
+    5-40 columns: 2P < m and fewer than 128 columns per pattern, so the
+    reference policy picks the single (not unrolled) pattern-reuse kernel
+    (cb-spmv-dcu.hpp:1431-1436, EXEC_PATTERN_REUSE_SINGLE)."""
+    rng = np.random.default_rng(seed)
+    sets = [np.sort(rng.choice(n, size=int(rng.integers(5, 41)), replace=False)) for _ in range(patterns)]
+    pick = rng.integers(0, patterns, size=m)
+    r = np.repeat(np.arange(m, dtype=np.int32), [sets[p].size for p in pick])
+    c = np.concatenate([sets[p] for p in pick]).astype(np.int32)
+    return m, n, r, c, np.ones(r.size)
+
+
+def test_repeated_patterns_policy_is_single_pattern():
+    """CPU: the fixture below takes the single pattern-reuse kernel, as the
+    oracle's restatement of the policy says, with P = the distinct row sets."""
+    m, n, r, c, v = repeated_patterns()
+    plan = Plan(m, n, r, c, v)
+    ob = pyoracle.OracleBuild(m, n, r, c, v)
+    pol = plan.export("policy").tolist()
+    want = ob.policy()
+    assert pol[:3] == [1, 1, 0]            # unit values, pattern reuse, not unrolled
+    assert want["use_pattern"] == 1 and want["use_unrolled"] == 0
+    distinct = {tuple(c[r == i]) for i in range(m)}
+    assert pol[7] == len(distinct) and 2 * pol[7] < m
+    assert plan.schedule() == "blocked"
+    plan.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("opts", [dict(), dict(row_owned=False), dict(block_size=32)], ids=["ro", "lpt", "bs32"])
+def test_single_pattern_kernel_parity(gpu, opts):
+    import torch
+
+    m, n, r, c, v = repeated_patterns()
+    print(f"case repeated_patterns/{opts}", flush=True)
+    plan = Plan(m, n, r, c, v, **opts)
+    assert plan.export("policy").tolist()[:3] == [1, 1, 0]
+    for kind in ("uniform", "integer"):
+        rng = np.random.default_rng(len(kind))
+        x = rng.uniform(-1, 1, n) if kind == "uniform" else rng.integers(-8, 9, n).astype(np.float64)
+        y_ref = pyoracle.spmv_seq(m, r, c, v, x)
+        xd = torch.tensor(x, dtype=torch.float64, device=gpu)
+        for sched in ("blocked", "stream"):
+            plan.set_schedule(sched)
+            y = plan.spmv(xd).cpu().numpy()
+            if kind == "integer":
+                assert np.array_equal(y, y_ref), f"{sched}: integer x not exact"
+            else:
+                assert pyoracle.rel_l2(y, y_ref) <= TOL, sched
+    plan.close()

@@ -93,3 +93,29 @@ def test_full_size_reduced_precision(gpu, config2, dtype, comp):
     p.close()
     tol = 1e-6 if dtype == "f32" else 2.0 ** -8
     assert pyoracle.rel_l2(y, y_exact) <= tol
+
+
+@pytest.mark.gpu
+@pytest.mark.slow
+def test_full_size_bs32_both_schedules(gpu, config2):
+    """The 32x32 build, where CSR blocks appear (coo2block.h:154-380 selects
+    them for blocks of more than 64 entries, here the power-law half's
+    near-diagonal blocks), on both schedules: the blocked schedule runs the
+    CSR branch of the per-block kernel (cb-spmv-dcu.hpp:881-952)."""
+    import torch
+
+    m, n, r, c, v, x = config2
+    p = Plan(m, n, r, c, v, block_size=32)
+    st = p.stats()
+    print(f"config2 bs32: {st['gb_nnzb']} blocks, COO {st['coo_blocks']} CSR {st['csr_blocks']} "
+          f"DENSE {st['dense_blocks']}", flush=True)
+    assert st["csr_blocks"] > 0
+    y_ref = pyoracle.spmv_csr_omp(m, r, c, v, x, THREADS)
+    xd = torch.tensor(x, device=gpu)
+    for sched in ("stream", "blocked"):
+        p.set_schedule(sched)
+        y = p.spmv(xd).cpu().numpy()
+        err = pyoracle.rel_l2(y, y_ref)
+        print(f"config2 bs32 {sched}: rel_l2 {err:.3e}", flush=True)
+        assert err <= 1e-12, (sched, err)
+    p.close()
