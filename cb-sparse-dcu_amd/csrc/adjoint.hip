@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "cb_runtime.hpp"
@@ -57,7 +58,38 @@ __global__ void sddmm_direct(const int* row, const int* col, const A* g, const A
 
 }  // namespace dev
 
+// Scratch of cbspmv_backward_stored_noalloc: the dx output the fused pass
+// writes when the caller passes no dx buffer.  A file-scope device array,
+// so the loader places one copy on every GPU the library runs on and keeps
+// it until the process ends; no allocation call is made for it.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
 namespace {
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.  The host-side name `scratch` is a
+// stand-in, not a device address: only the looked-up pointer may be passed
+// to kernels.
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;
+
+int scratch_on_current_device(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(CBSPMV_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return CBSPMV_OK;
+}
 
 int grid_for(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192)); }
 
@@ -157,6 +189,30 @@ int cbspmv_backward_stored(cbspmv_plan* plan, const void* d_g, const void* d_x, 
     void* dx = d_dx;
     if (!dx && (rc = cbs::plan_scratch(plan->dev.dvals_stored, (size_t)th.m * cbs::vec_elem_bytes(th), s, &dx)))
         return rc;
+    return cbs::run_stream(*t, d_g, dx, s, d_x, d_dvals_stored);
+}
+
+int cbspmv_backward_stored_noalloc(cbspmv_plan* plan, const void* d_g, const void* d_x, void* d_dx,
+                                   void* d_dvals_stored, void* stream) {
+    if (!plan) return fail(CBSPMV_ERR_ARG, "null plan");
+    cbspmv_plan* t = plan->transpose;
+    if (!t) return fail(CBSPMV_ERR_STATE, "no transposed plan (call cbspmv_build_transpose first)");
+    const cbs::Plan& th = t->host;
+    if (th.nnz > 0 && (!d_g || !d_x || !d_dvals_stored)) return fail(CBSPMV_ERR_ARG, "null vector");
+    if (reinterpret_cast<uintptr_t>(d_dvals_stored) % 16) return fail(CBSPMV_ERR_ARG, "dvals must be 16-byte aligned");
+    const size_t dx_bytes = (size_t)th.m * cbs::vec_elem_bytes(th);
+    if (!d_dx && dx_bytes > cbs::kNoAllocScratchBytes)
+        return fail(CBSPMV_ERR_ARG, "n times the vector element size is above the 16 MiB scratch: pass a dx buffer");
+    int dev = -1;
+    if (!plan->dev.uploaded || !t->dev.uploaded || hipGetDevice(&dev) != hipSuccess || dev != plan->dev.device ||
+        dev != t->dev.device)
+        return fail(CBSPMV_ERR_STATE, "the plan and its transpose must be uploaded to the current device "
+                                      "(cbspmv_upload after cbspmv_build_transpose)");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (th.nnz == 0 || th.m == 0) return d_dx && th.m ? cbs::run_spmv(*t, d_g, d_dx, s) : CBSPMV_OK;
+    void* dx = d_dx;
+    int rc;
+    if (!dx && (rc = cbs::scratch_on_current_device(&dx))) return rc;
     return cbs::run_stream(*t, d_g, dx, s, d_x, d_dvals_stored);
 }
 

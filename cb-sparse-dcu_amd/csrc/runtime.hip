@@ -276,7 +276,10 @@ void cbspmv_destroy(cbspmv_plan* plan) {
 
 int cbspmv_upload(cbspmv_plan* plan, void* stream) {
     if (!plan) return fail(CBSPMV_ERR_ARG, "null plan");
-    return cbs::upload_plan(*plan, static_cast<hipStream_t>(stream));
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const int rc = cbs::upload_plan(*plan, s);
+    if (rc || !plan->transpose) return rc;
+    return cbs::upload_plan(*plan->transpose, s);
 }
 
 int cbspmv_spmv(cbspmv_plan* plan, const void* d_x, void* d_y, void* stream) {

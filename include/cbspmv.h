@@ -151,7 +151,8 @@ int cbspmv_save(const cbspmv_plan* plan, const char* path);
 int cbspmv_load(const char* path, cbspmv_plan** plan);
 
 /* ---- device execution (stream-ordered on the caller's stream) --------------- */
-/* Copies the plan's device slabs to the current device (allocates). */
+/* Copies the plan's device slabs to the current device (allocates); also
+ * the transposed plan's, when cbspmv_build_transpose has built it. */
 int cbspmv_upload(cbspmv_plan* plan, void* stream);
 /* y = A x.  d_x has n elements, d_y has m elements, both of the plan's
  * vector type: double for F64 plans, float for F32/BF16 plans.  y is
@@ -197,6 +198,20 @@ int cbspmv_backward(cbspmv_plan* plan, const void* d_g, const void* d_x,
  * hold no input entry (zero fill of dense blocks, padding) get g_i x_j or 0. */
 int cbspmv_backward_stored(cbspmv_plan* plan, const void* d_g, const void* d_x,
                            void* d_dx, void* d_dvals_stored, void* stream);
+/* cbspmv_backward_stored that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory during the call
+ * (HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Same outputs as cbspmv_backward_stored (bit for bit); differences:
+ *   - the plan and its transpose must already be on the current device
+ *     (cbspmv_upload uploads the transpose too once it is built), else
+ *     CBSPMV_ERR_STATE;
+ *   - when d_dx is NULL the pass writes dx into a library-owned device
+ *     array of 16 MiB per GPU (held until the process ends), so then
+ *     n * element size must be at most 16 MiB (2M FP64 or 4M FP32
+ *     columns), else CBSPMV_ERR_ARG.  Calls that overlap in time on one GPU
+ *     share that array; its contents are discarded. */
+int cbspmv_backward_stored_noalloc(cbspmv_plan* plan, const void* d_g, const void* d_x,
+                                   void* d_dx, void* d_dvals_stored, void* stream);
 /* Replaces the values of A on the device: d_vals holds nnz values in input
  * COO order, of the plan's vector type (double for F64 plans, float for F32
  * and BF16 plans, rounded to the storage type).  Duplicates are summed in
