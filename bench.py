@@ -265,6 +265,18 @@ def bench_precision(args):
 
 
 def bench_autograd(args):
+    """fwd+bwd / fwd of the differentiable SpMV through its torch.autograd
+    wrapper: y = cb_spmv(A, x) with A.values (stored order) and x both
+    requiring gradients, then y.backward(g) (dx = A^T g and the gradient of
+    every stored value, one fused pass over A^T).  fwd is the plain forward
+    launch.  `value` times both inside the training step (events before,
+    between and after them, so each runs after the other as in training);
+    also reported: the step as a captured graph replay and eagerly, the
+    forward of an inference loop (each launch after another forward, the
+    matrix partly cached), the kernels alone, the value push after an
+    optimizer step, and the input-order alternatives."""
+    import ctypes
+
     import numpy as np
     import torch
 
@@ -281,49 +293,109 @@ def bench_autograd(args):
     stream = torch.cuda.current_stream()
     rng = np.random.default_rng(7)
     g64 = rng.uniform(-1, 1, m)
-    x = torch.tensor(d["x"], dtype=torch.float32, device=dev)
+    x = torch.tensor(d["x"], dtype=torch.float32, device=dev, requires_grad=True)
     g = torch.tensor(g64, dtype=torch.float32, device=dev)
     threads = min(16, os.cpu_count() or 1)
+    xd, gd = d["x"].astype(np.float32).astype(np.float64), g.double().cpu().numpy()
     o = np.argsort(c, kind="stable")
-    dx64 = pyoracle.spmv_csr_omp(n, c[o], r[o], v[o], g64, threads)
-    dv64 = g64[r] * d["x"][c]
-    res = {}
-    for mode in ("fused", "direct"):
-        plan = cbspmv.Plan(m, n, r, c, v, dtype="f32", backward=mode)
-        t1 = time.perf_counter()
-        plan.build_transpose()
-        log(f"{mode}: transposed plan built in {time.perf_counter() - t1:.1f}s")
-        plan.upload(stream)
-        y = torch.empty(m, dtype=torch.float32, device=dev)
-        dx = torch.empty(n, dtype=torch.float32, device=dev)
-        dv = torch.empty(nnz, dtype=torch.float32, device=dev)
-        lib = cbspmv.lib()
-        import ctypes
+    dx64 = pyoracle.CSR(n, c[o], r[o], v[o]).spmv(gd, threads)
+    dv64 = gd[r] * xd[c]
 
-        h, sh = plan.handle, ctypes.c_void_p(stream.cuda_stream)
-        P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
-        fwd = timed(lambda: plan.spmv(x, y, stream), args.steps, args.warmup, stream)
-        both = timed(lambda: lib.cbspmv_backward(h, P(g), P(x), P(dx), P(dv), sh), args.steps, args.warmup, stream)
-        err_dx = pyoracle.rel_l2(dx.double().cpu().numpy(), dx64)
-        err_dv = pyoracle.rel_l2(dv.double().cpu().numpy(), dv64)
-        only_dx = timed(lambda: lib.cbspmv_backward(h, P(g), P(x), P(dx), ctypes.c_void_p(0), sh), args.steps,
-                        args.warmup, stream)
-        entry = {"fwd_us": round(fwd * 1e3, 2), "bwd_dx_dvals_us": round(both * 1e3, 2),
-                 "bwd_dx_only_us": round(only_dx * 1e3, 2), "ratio_fwd_bwd_over_fwd": round((fwd + both) / fwd, 3),
-                 "ratio_dx_only": round((fwd + only_dx) / fwd, 3), "rel_l2_dx": err_dx, "rel_l2_dvals": err_dv}
-        if mode == "fused":
-            st = torch.empty(plan.stored_entries(), dtype=torch.float32, device=dev)
-            stored = timed(lambda: lib.cbspmv_backward_stored(h, P(g), P(x), P(dx), P(st), sh), args.steps,
-                           args.warmup, stream)
-            entry["bwd_stored_order_us"] = round(stored * 1e3, 2)
-            entry["ratio_stored_order"] = round((fwd + stored) / fwd, 3)
-        res[mode] = entry
-        log(f"{mode}: {entry}")
+    t1 = time.perf_counter()
+    A = cbspmv.CBSparse(m, n, r, c, v, dtype="f32", requires_grad=True)
+    log(f"CBSparse (A and A^T) built in {time.perf_counter() - t1:.1f}s, {A.stored} stored entries")
+    y = torch.empty(m, dtype=torch.float32, device=dev)
+    xs = x.detach()
+    fwd = timed(lambda: A.plan.spmv(xs, y, stream), args.steps, args.warmup, stream)
+
+    def step():
+        A.values.grad = None
+        x.grad = None
+        cbspmv.cb_spmv_autograd(A, x).backward(g)
+
+    eager = timed(step, args.steps, args.warmup, stream)
+    err_dx = pyoracle.rel_l2(x.grad.double().cpu().numpy(), dx64)
+    err_dv = pyoracle.rel_l2(A.to_input(A.values.grad).double().cpu().numpy(), dv64)
+    graph_ms = None
+    try:
+        side = torch.cuda.Stream()
+        side.wait_stream(stream)
+        with torch.cuda.stream(side):
+            for _ in range(3):
+                step()
+        stream.wait_stream(side)
+        torch.cuda.synchronize()
+        cg = torch.cuda.CUDAGraph()
+        A.values.grad = None
+        x.grad = None
+        with torch.cuda.graph(cg):
+            cbspmv.cb_spmv_autograd(A, x).backward(g)
+        graph_ms = timed(cg.replay, args.steps, args.warmup, stream)
+        err_graph = pyoracle.rel_l2(A.to_input(A.values.grad).double().cpu().numpy(), dv64)
+    except Exception as e:   # capture is a measurement aid; the eager step stays the reference
+        log(f"graph capture of the autograd step failed: {e}")
+        err_graph = None
+
+    # the forward and the backward inside the training step: events before,
+    # between and after, on the launch stream (each kernel then runs where a
+    # training loop runs it, after the other one)
+    marks = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    for e in marks:
+        A.values.grad = None
+        x.grad = None
+        e[0].record(stream)
+        yy = cbspmv.cb_spmv_autograd(A, x)
+        e[1].record(stream)
+        yy.backward(g)
+        e[2].record(stream)
+        del yy   # frees the step's autograd graph
+    torch.cuda.synchronize()
+    fwd_step = sum(e[0].elapsed_time(e[1]) for e in marks) / len(marks)
+    bwd_step = sum(e[1].elapsed_time(e[2]) for e in marks) / len(marks)
+    # the kernels alone
+    lib = cbspmv.lib()
+    h, sh = A.plan.handle, ctypes.c_void_p(stream.cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    dx = torch.empty(n, dtype=torch.float32, device=dev)
+    dw = torch.empty(A.stored, dtype=torch.float32, device=dev)
+    bwd = timed(lambda: lib.cbspmv_backward_stored(h, P(g), P(xs), P(dx), P(dw), sh), args.steps, args.warmup,
+                stream)
+    # pushing updated values (after an optimizer step) to A and A^T
+    w = A.values.detach().clone()
+    push = timed(lambda: A.plan.set_values_stored(w, stream), max(5, args.steps // 4), 2, stream)
+    # input-order alternatives (kernel level)
+    vin = torch.tensor(v.astype(np.float32), device=dev)
+    dvin = torch.empty(nnz, dtype=torch.float32, device=dev)
+    alt = {}
+    for mode in ("direct", "fused"):
+        plan = cbspmv.Plan(m, n, r, c, v, dtype="f32", backward=mode)
+        plan.build_transpose()
+        plan.upload(stream)
+        hh = plan.handle
+        t = timed(lambda: lib.cbspmv_backward(hh, P(g), P(xs), P(dx), P(dvin), sh), args.steps, args.warmup, stream)
+        alt[f"input_order_{mode}_bwd_us"] = round(t * 1e3, 2)
         plan.close()
-    fused = res["fused"]
-    return {"metric": "fwd+bwd / fwd time, differentiable SpMV (dx = A^T g and dvals over the stored pattern)",
-            "value": fused["ratio_stored_order"], "unit": "x", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": (fused["fwd_us"] + fused["bwd_stored_order_us"]) / 1e3,
+    push_in = timed(lambda: A.plan.set_values(vin, stream), max(5, args.steps // 4), 2, stream)
+    A.close()
+    ratio = (fwd_step + bwd_step) / fwd_step
+    res = {"fwd_us": round(fwd * 1e3, 2),
+           "fwd_in_training_step_us": round(fwd_step * 1e3, 2), "bwd_in_training_step_us": round(bwd_step * 1e3, 2),
+           "ratio_in_training_step": round(ratio, 3),
+           "wrapper_step_graph_us": round(graph_ms * 1e3, 2) if graph_ms is not None else None,
+           "wrapper_step_eager_us": round(eager * 1e3, 2),
+           "ratio_graph_over_inference_fwd": round(graph_ms / fwd, 3) if graph_ms is not None else None,
+           "ratio_eager_with_host_gaps": round(eager / fwd, 3),
+           "bwd_kernel_us": round(bwd * 1e3, 2), "ratio_kernels": round((fwd + bwd) / fwd, 3),
+           "value_push_stored_us": round(push * 1e3, 2), "value_push_input_order_us": round(push_in * 1e3, 2),
+           "rel_l2_dx": err_dx, "rel_l2_dvals": err_dv, "rel_l2_dvals_graph": err_graph, **alt}
+    log(f"autograd: {res}")
+    return {"metric": "fwd+bwd / fwd time of the differentiable SpMV through torch.autograd "
+                      "(dx = A^T g and dvals of every stored value), both timed inside the training step",
+            "value": round(ratio, 3), "unit": "x", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": fwd_step + bwd_step,
             "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
             "data": "synthetic (seeded generator, cbspmv/gen.py config5)",
             "config": {"workload": "powerlaw_512k_20M_nnz_autograd", "m": m, "n": n, "nnz": nnz}, "results": res}

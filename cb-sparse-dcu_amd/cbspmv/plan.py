@@ -216,6 +216,15 @@ class Plan:
         self._check_vec(vals, self.nnz, "vals")
         check(lib().cbspmv_set_values(self._h, ctypes.c_void_p(vals.data_ptr()), _stream_handle(stream)))
 
+    def set_values_stored(self, vals, stream=None):
+        """Replaces A's values from stored order (stored_entries() values,
+        the order of backward_stored's dvals)."""
+        if not self._uploaded:
+            self.upload(stream)
+        self._check_vec(vals, self.stored_entries(), "vals")
+        f = _lib.bind("cbspmv_set_values_stored", ctypes.c_int, [ctypes.c_void_p] * 3)
+        check(f(self._h, ctypes.c_void_p(vals.data_ptr()), _stream_handle(stream)))
+
     # ---- persistence
     def save(self, path):
         """Writes the built plan to a file (cbspmv_save)."""

@@ -120,29 +120,75 @@ int ensure_groups(cbspmv_plan& p, const std::vector<uint32_t>& slot, hipStream_t
     return CBSPMV_OK;
 }
 
-int launch_set_values(cbspmv_plan& p, const void* vals, hipStream_t s) {
+// Stored-order groups: values come as one per stored entry (the transposed
+// plan's padded stream order).  Every stored entry that holds input entries
+// is owned by the stream entry of p's plan that holds its first input entry
+// (lowest input index); a group sums the stored entries it owns.  For the
+// transposed plan itself that is the identity (each group reads its own
+// stored entry).
+int ensure_stored_groups(cbspmv_plan& p, const std::vector<uint32_t>& slot, const std::vector<uint32_t>& grad_slot,
+                         bool identity, hipStream_t s) {
     DevicePlan& d = p.dev;
-    if (d.sv_groups == 0) return CBSPMV_OK;
-    const int grid = grid_for(d.sv_groups);
-    const auto* pos = static_cast<const uint32_t*>(d.sv_pos.ptr);
-    const auto* ptr = static_cast<const uint32_t*>(d.sv_ptr.ptr);
-    const auto* src = static_cast<const uint32_t*>(d.sv_src.ptr);
+    if (d.ss_pos.ptr || p.host.nnz == 0) return CBSPMV_OK;
+    std::vector<uint32_t> pos, ptr, src;
+    int rc = value_groups(p.host, slot, pos, ptr, src);
+    if (rc) return rc;
+    std::vector<uint32_t> sptr(1, 0), ssrc;
+    ssrc.reserve(pos.size());
+    if (identity) {
+        // slot == grad_slot here: a group's input entries share one stored entry
+        for (size_t g = 0; g < pos.size(); ++g) {
+            ssrc.push_back(grad_slot[src[ptr[g]]]);
+            sptr.push_back((uint32_t)ssrc.size());
+        }
+    } else {
+        size_t stored = 0;
+        for (uint32_t e : grad_slot) stored = std::max<size_t>(stored, (size_t)e + 1);
+        std::vector<uint32_t> owner(stored, 0xffffffffu);
+        for (size_t k = grad_slot.size(); k-- > 0;) owner[grad_slot[k]] = (uint32_t)k;   // lowest k wins
+        for (size_t g = 0; g < pos.size(); ++g) {
+            for (uint32_t q = ptr[g]; q < ptr[g + 1]; ++q) {
+                const uint32_t k = src[q];
+                if (owner[grad_slot[k]] == k) ssrc.push_back(grad_slot[k]);
+            }
+            sptr.push_back((uint32_t)ssrc.size());
+        }
+    }
+    if ((rc = upload_vec(d.ss_pos, pos, s)) || (rc = upload_vec(d.ss_ptr, sptr, s)) || (rc = upload_vec(d.ss_src, ssrc, s)))
+        return rc;
+    d.ss_groups = (uint32_t)pos.size();
+    return CBSPMV_OK;
+}
+
+int launch_group_values(cbspmv_plan& p, const void* vals, const DeviceBuf& gpos, const DeviceBuf& gptr,
+                        const DeviceBuf& gsrc, uint32_t groups, hipStream_t s) {
+    DevicePlan& d = p.dev;
+    if (groups == 0) return CBSPMV_OK;
+    const int grid = grid_for(groups);
+    const auto* pos = static_cast<const uint32_t*>(gpos.ptr);
+    const auto* ptr = static_cast<const uint32_t*>(gptr.ptr);
+    const auto* src = static_cast<const uint32_t*>(gsrc.ptr);
     auto* slab = static_cast<uint8_t*>(d.key.ptr);
     switch (p.host.o.dtype) {
         case CBSPMV_F64:
             hipLaunchKernelGGL((dev::set_values<double, CBSPMV_F64>), dim3(grid), dim3(256), 0, s,
-                               static_cast<const double*>(vals), pos, ptr, src, d.sv_groups, slab);
+                               static_cast<const double*>(vals), pos, ptr, src, groups, slab);
             break;
         case CBSPMV_F32:
             hipLaunchKernelGGL((dev::set_values<float, CBSPMV_F32>), dim3(grid), dim3(256), 0, s,
-                               static_cast<const float*>(vals), pos, ptr, src, d.sv_groups, slab);
+                               static_cast<const float*>(vals), pos, ptr, src, groups, slab);
             break;
         default:
             hipLaunchKernelGGL((dev::set_values<float, CBSPMV_BF16>), dim3(grid), dim3(256), 0, s,
-                               static_cast<const float*>(vals), pos, ptr, src, d.sv_groups, slab);
+                               static_cast<const float*>(vals), pos, ptr, src, groups, slab);
     }
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? CBSPMV_OK : hip_fail(e, "set_values launch");
+}
+
+int launch_set_values(cbspmv_plan& p, const void* vals, hipStream_t s) {
+    const DevicePlan& d = p.dev;
+    return launch_group_values(p, vals, d.sv_pos, d.sv_ptr, d.sv_src, d.sv_groups, s);
 }
 
 // A device buffer of at least `bytes` owned by the plan (grown on demand).
@@ -273,6 +319,29 @@ int cbspmv_backward(cbspmv_plan* plan, const void* d_g, const void* d_x, void* d
                            reinterpret_cast<const float*>(buf), idx, static_cast<float*>(d_dvals), (uint32_t)h.nnz);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? CBSPMV_OK : cbs::hip_fail(e, "gather launch");
+}
+
+int cbspmv_set_values_stored(cbspmv_plan* plan, const void* d_vals_stored, void* stream) {
+    if (!plan) return fail(CBSPMV_ERR_ARG, "null plan");
+    if (!plan->transpose) return fail(CBSPMV_ERR_STATE, "no transposed plan (call cbspmv_build_transpose first)");
+    if (!plan->dev.uploaded) return fail(CBSPMV_ERR_STATE, "plan not uploaded (call cbspmv_upload first)");
+    if (plan->host.nnz == 0) return CBSPMV_OK;
+    if (!d_vals_stored) return fail(CBSPMV_ERR_ARG, "null values");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc;
+    cbspmv_plan* t = cbs::ready_transpose(plan, s, rc);
+    if (!t) return rc;
+    if ((rc = cbs::ensure_stored_groups(*plan, plan->slot_of, plan->grad_slot, false, s)) ||
+        (rc = cbs::ensure_stored_groups(*t, plan->grad_slot, plan->grad_slot, true, s)))
+        return rc;
+    const cbs::DevicePlan& d = plan->dev;
+    const cbs::DevicePlan& td = t->dev;
+    if ((rc = cbs::launch_group_values(*plan, d_vals_stored, d.ss_pos, d.ss_ptr, d.ss_src, d.ss_groups, s)) ||
+        (rc = cbs::launch_group_values(*t, d_vals_stored, td.ss_pos, td.ss_ptr, td.ss_src, td.ss_groups, s)))
+        return rc;
+    plan->values_replaced = true;
+    t->values_replaced = true;
+    return CBSPMV_OK;
 }
 
 int cbspmv_set_values(cbspmv_plan* plan, const void* d_vals, void* stream) {

@@ -35,6 +35,10 @@ struct DevicePlan {
     // byte offset in the stream slab and its input entries (CSR, input order)
     DeviceBuf sv_pos, sv_ptr, sv_src;
     uint32_t sv_groups = 0;
+    // the same for values given in stored order (the transposed plan's
+    // stream order): each group's sources are stored entries
+    DeviceBuf ss_pos, ss_ptr, ss_src;
+    uint32_t ss_groups = 0;
     // backward (forward plan only): input entry -> transposed stream index,
     // the stored-order dvals buffer, and the input coordinates
     DeviceBuf grad_slot, dvals_stored, in_row, in_col;

@@ -39,10 +39,13 @@ size_t vec_elem_bytes(const Plan& h) { return h.o.dtype == CBSPMV_F64 ? 8 : 4; }
 void release_device(DevicePlan& d) {
     for (DeviceBuf* b : {&d.slab, &d.zero_strips, &d.key, &d.val, &d.items, &d.item_split,
                          &d.item_split_first, &d.splits, &d.partial, &d.counters, &d.empty, &d.tiles, &d.tile_lines,
-                         &d.sv_pos, &d.sv_ptr, &d.sv_src, &d.grad_slot, &d.dvals_stored, &d.in_row, &d.in_col,
+                         &d.sv_pos, &d.sv_ptr, &d.sv_src, &d.ss_pos, &d.ss_ptr, &d.ss_src, &d.grad_slot,
+                         &d.dvals_stored, &d.in_row, &d.in_col,
                          &d.panel_items, &d.panel_mask, &d.panel_out})
         release(*b);
     d.uploaded = false;
+    d.sv_groups = 0;
+    d.ss_groups = 0;
 }
 
 int choose_schedule(const cbspmv_plan& p) {

@@ -243,6 +243,39 @@ __device__ __forceinline__ void join_pieces(const StreamArgs<V, A>& a, uint32_t 
     }
 }
 
+// SDDMM products dv[j] = xc[j] * xrow[row of entry j] for one chunk.  A
+// chunk's rows are few and consecutive: lane l loads xrow[r0 + l] once and
+// entries take their row's value by a lane shuffle.  A lane's entries
+// usually span one or two rows, its first entry's and its last entry's, so
+// two shuffles serve them all; a lane with an entry in a row in between, or
+// in a row 64 or more past r0, redoes its entries one shuffle (or load)
+// each.  Every shuffle is issued by the whole wave.
+template <typename A, int EPL>
+__device__ __forceinline__ void row_products(const A* xrow, uint32_t m, uint32_t row_base, const int (&off)[EPL],
+                                             bool valid, const A (&xc)[EPL], A (&dv)[EPL]) {
+    const int lane = lane_id();
+    const uint32_t r0 = row_base + (uint32_t)__builtin_amdgcn_readfirstlane(off[0]);
+    const A xw = xrow[min(r0 + (uint32_t)lane, m - 1u)];
+    const uint32_t dlo = row_base + (uint32_t)off[0] - r0, dhi = row_base + (uint32_t)off[EPL - 1] - r0;
+    const A xa = __shfl(xw, valid && dlo < 64u ? (int)dlo : 0, kWave);
+    const A xb = __shfl(xw, valid && dhi < 64u ? (int)dhi : 0, kWave);
+    bool slow = valid && (dlo >= 64u || dhi >= 64u);
+#pragma unroll
+    for (int j = 0; j < EPL; ++j) {
+        const bool lo = off[j] == off[0];
+        dv[j] = xc[j] * (lo ? xa : xb);
+        slow = slow || (valid && !lo && off[j] != off[EPL - 1]);
+    }
+    if (__any(slow)) {
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+            const uint32_t d = row_base + (uint32_t)off[j] - r0;
+            const A v = __shfl(xw, valid && d < 64u ? (int)d : 0, kWave);
+            if (slow) dv[j] = xc[j] * (d < 64u ? v : xrow[min(row_base + (uint32_t)off[j], m - 1u)]);
+        }
+    }
+}
+
 // Chunks in flight per wave (the ring of stream_item): 5 when a chunk holds
 // 12 registers (FP64, BF16), 4 when it holds 16 (FP32); 3 where registers
 // are short (the batched kernel, FP32 backward).  The ring pays off when x is
@@ -323,27 +356,8 @@ __device__ __forceinline__ void stream_item_near(const StreamArgs<V, A>& a, uint
             else s[j] = A(widen(cur.v[j])) * xc[j];
         }
         if constexpr (SDDMM) {
-            // x of each entry's row: lane l loads x[r0 + l] once and each
-            // entry takes its row's value by a lane shuffle; rows 64 or more
-            // past r0 load directly
-            const uint32_t r0 = it.row_base + (uint32_t)__builtin_amdgcn_readfirstlane(off[0]);
-            const A xw = a.xrow[min(r0 + (uint32_t)lane, (uint32_t)a.m - 1u)];
             A dv[EPL];
-            bool far = false;
-#pragma unroll
-            for (int j = 0; j < EPL; ++j) {
-                const uint32_t d = it.row_base + (uint32_t)off[j] - r0;
-                const bool near = valid && d < 64u;
-                far = far || (valid && !near);
-                dv[j] = xc[j] * __shfl(xw, near ? (int)d : 0, kWave);
-            }
-            if (__any(far)) {
-#pragma unroll
-                for (int j = 0; j < EPL; ++j) {
-                    const uint32_t d = it.row_base + (uint32_t)off[j] - r0;
-                    if (valid && d >= 64u) dv[j] = xc[j] * a.xrow[min(it.row_base + (uint32_t)off[j], (uint32_t)a.m - 1u)];
-                }
-            }
+            row_products<A, EPL>(a.xrow, (uint32_t)a.m, it.row_base, off, valid, xc, dv);
             if (valid) {
                 u32x4* dst = reinterpret_cast<u32x4*>(a.dvals + e0 + (uint32_t)lane * EPL);
 #pragma unroll
@@ -472,27 +486,8 @@ __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t 
             else s[j] = A(widen(c.v[j])) * xc[j];
         }
         if constexpr (SDDMM) {
-            // x of each entry's row: a chunk's rows are few and consecutive,
-            // so lane l loads x[r0 + l] once and each entry takes its row's
-            // value by a lane shuffle; rows 64 or more past r0 load directly
-            const uint32_t rr = it.row_base + (uint32_t)__builtin_amdgcn_readfirstlane(off[0]);
-            const A xw = a.xrow[min(rr + (uint32_t)lane, (uint32_t)a.m - 1u)];
             A dv[EPL];
-            bool far = false;
-#pragma unroll
-            for (int j = 0; j < EPL; ++j) {
-                const uint32_t d = it.row_base + (uint32_t)off[j] - rr;
-                const bool near = valid && d < 64u;
-                far = far || (valid && !near);
-                dv[j] = xc[j] * __shfl(xw, near ? (int)d : 0, kWave);
-            }
-            if (__any(far)) {
-#pragma unroll
-                for (int j = 0; j < EPL; ++j) {
-                    const uint32_t d = it.row_base + (uint32_t)off[j] - rr;
-                    if (valid && d >= 64u) dv[j] = xc[j] * a.xrow[min(it.row_base + (uint32_t)off[j], (uint32_t)a.m - 1u)];
-                }
-            }
+            row_products<A, EPL>(a.xrow, (uint32_t)a.m, it.row_base, off, valid, xc, dv);
             const rsrc_t rd = make_rsrc(a.dvals + it.e_begin, ne * (uint32_t)sizeof(A));
             const uint32_t at = valid ? (r0 + (uint32_t)lane * EPL) * (uint32_t)sizeof(A) : kOutOfRange;
 #pragma unroll

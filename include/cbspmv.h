@@ -220,6 +220,16 @@ int cbspmv_backward_stored_noalloc(cbspmv_plan* plan, const void* d_g, const voi
  * the build-time values. */
 int cbspmv_set_values(cbspmv_plan* plan, const void* d_vals, void* stream);
 
+/* cbspmv_set_values with the values given in stored order: d_vals_stored
+ * holds cbspmv_stored_entries() values (the order of cbspmv_backward_stored's
+ * dvals), so a trainer can keep its parameters in the order its gradients
+ * come in.  Stored entries that hold no input entry are ignored; a stored
+ * entry that merges duplicate input entries carries their sum.  Both plans
+ * are updated: A^T from its own entries, A by summing, per stream entry,
+ * the stored entries whose first input entry it holds.  Needs
+ * cbspmv_build_transpose and cbspmv_upload. */
+int cbspmv_set_values_stored(cbspmv_plan* plan, const void* d_vals_stored, void* stream);
+
 /* ---- batching (no reference counterpart; extension) -------------------------- */
 typedef struct cbspmv_batch cbspmv_batch;
 /* Groups `count` built plans (borrowed; they must outlive the batch) of one

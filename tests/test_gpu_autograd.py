@@ -258,3 +258,146 @@ def test_two_value_sets_share_one_matrix(gpu):
     with pytest.raises(RuntimeError):
         y3.sum().backward()
     A.close()
+
+
+# ---- trainable values in stored order (CBSparse.values) -----------------------
+
+def _sparse_ref(r, c, v, x0, weights):
+    """y, x.grad and per-input-entry value gradients of sum(sin(y) * w) by
+    torch.sparse on the CPU in FP64."""
+    import torch
+
+    tv = torch.tensor(v, dtype=torch.float64, requires_grad=True)
+    tx = torch.tensor(x0, dtype=torch.float64, requires_grad=True)
+    S = torch.sparse_coo_tensor(torch.tensor(np.stack([r, c]), dtype=torch.long), tv, (M, N))
+    ty = torch.sparse.mm(S, tx.view(-1, 1)).view(-1)
+    (torch.sin(ty) * torch.tensor(weights)).sum().backward()
+    return ty.detach().numpy(), tx.grad.numpy(), tv.grad.numpy()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dup", [False, True], ids=["nodup", "dup"])
+def test_stored_values_match_torch_sparse(gpu, dup):
+    """cb_spmv(A, x) differentiates A.values (stored order) with the fused
+    pass alone; mapped to input order the gradient is g_i x_j per entry."""
+    import torch
+
+    r, c, v = _matrix(seed=21, dup=dup)
+    A = CBSparse(M, N, r, c, v, requires_grad=True)
+    assert A.values.numel() == A.stored
+    x0 = np.random.default_rng(22).uniform(-1, 1, N)
+    w = np.arange(M, dtype=np.float64)
+    x = _t(x0, gpu).requires_grad_(True)
+    y = cb_spmv_autograd(A, x)
+    (torch.sin(y) * _t(w, gpu)).sum().backward()
+    ty, txg, tvg = _sparse_ref(r, c, v, x0, w)
+    assert pyoracle.rel_l2(y.detach().cpu().numpy(), ty) <= 1e-12
+    assert pyoracle.rel_l2(x.grad.cpu().numpy(), txg) <= 1e-12
+    # each input entry's gradient is its stored entry's (duplicates share one)
+    assert pyoracle.rel_l2(A.to_input(A.values.grad).cpu().numpy(), tvg) <= 1e-12
+    # stored entries holding no input entry (padding) get a zero gradient
+    live = np.zeros(A.stored, bool)
+    live[A.plan.export("grad_slot")] = True
+    assert np.all(A.values.grad.cpu().numpy()[~live] == 0)
+    A.close()
+
+
+@pytest.mark.gpu
+def test_stored_values_finite_differences(gpu):
+    import torch
+
+    r, c, v = _matrix(seed=23)
+    A = CBSparse(M, N, r, c, v, requires_grad=True)
+    x0 = np.random.default_rng(24).uniform(-1, 1, N)
+    x = _t(x0, gpu)
+    loss = torch.sin(cb_spmv_autograd(A, x)).sum()
+    loss.backward()
+    grad = A.values.grad.cpu().numpy()
+    base = A.values.detach().clone()
+
+    def L(wv):
+        A.values = wv
+        with torch.no_grad():
+            return torch.sin(cb_spmv_autograd(A, x)).sum().item()
+
+    rng = np.random.default_rng(25)
+    h = 1e-5
+    live = np.zeros(A.stored, bool)
+    live[A.plan.export("grad_slot")] = True
+    for _ in range(3):
+        e = rng.standard_normal(A.stored) * live
+        et = _t(e, gpu)
+        fd = (L(base + h * et) - L(base - h * et)) / (2 * h)
+        an = float(grad @ e)
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an))
+    A.close()
+
+
+@pytest.mark.gpu
+def test_stored_values_training(gpu):
+    """Gradient descent on A.values for ||A x - b||^2 with a fixed x: each
+    step's loss and values match the same iteration in FP64 numpy; the next
+    forward pushes the updated values (cbspmv_set_values_stored)."""
+    import torch
+
+    r, c, v = _matrix(seed=27)      # no duplicates: stored order is a permutation of input order
+    A = CBSparse(M, N, r, c, v, requires_grad=True)
+    rng = np.random.default_rng(28)
+    x0 = rng.uniform(-1, 1, N)
+    v_star = v + 0.1 * rng.standard_normal(v.size)
+    b = sp.csr_matrix((v_star, (r, c)), shape=(M, N)) @ x0
+    x, bt = _t(x0, gpu), _t(b, gpu)
+    opt = torch.optim.SGD([A.values], lr=0.05)
+    vals = v.copy()
+    slot = A.plan.export("grad_slot").astype(np.int64)
+    losses = []
+    for step in range(20):
+        opt.zero_grad()
+        loss = ((cb_spmv_autograd(A, x) - bt) ** 2).sum()
+        loss.backward()
+        opt.step()
+        res = sp.csr_matrix((vals, (r, c)), shape=(M, N)) @ x0 - b
+        assert abs(loss.item() - res @ res) <= 1e-10 * max(res @ res, 1e-30), step
+        vals = vals - 0.05 * 2.0 * res[r] * x0[c]
+        got = A.values.detach().cpu().numpy()[slot]
+        assert pyoracle.rel_l2(got, vals) <= 1e-12, step
+        losses.append(loss.item())
+    assert losses[-1] < 0.5 * losses[0]
+    # after pushing the last step's values the plans compute with the trained
+    # values (forward and transpose)
+    A.sync_stored(A.values)
+    y = A.plan.spmv(x).cpu().numpy()
+    assert pyoracle.rel_l2(y, pyoracle.spmv_seq(M, r, c, vals, x0)) <= 1e-12
+    g = rng.uniform(-1, 1, M)
+    o = np.argsort(c, kind="stable")
+    dx = A.plan.spmv_transpose(_t(g, gpu)).cpu().numpy()
+    assert pyoracle.rel_l2(dx, pyoracle.spmv_seq(N, c[o], r[o], vals[o], g)) <= 1e-12
+    A.close()
+
+
+@pytest.mark.gpu
+def test_stored_values_dense_fill_masked(gpu):
+    """A matrix with dense blocks: A^T's zero fill entries get no gradient,
+    and the trained values stay consistent between A and A^T."""
+    import torch
+
+    rng = np.random.default_rng(29)
+    D = (rng.random((96, 96)) < 0.1) * rng.uniform(-1, 1, (96, 96))
+    D[16:32, 32:48] = rng.uniform(-1, 1, (16, 16))
+    D[20, 40] = 0.0                      # a hole: zero fill in the dense block
+    r, c = np.nonzero(D)
+    r, c = r.astype(np.int32), c.astype(np.int32)
+    v = D[r, c]
+    A = CBSparse(96, 96, r, c, v, requires_grad=True, builder="comp")
+    assert A._mask is not None
+    x0 = rng.uniform(-1, 1, 96)
+    x = _t(x0, gpu).requires_grad_(True)
+    y = cb_spmv_autograd(A, x)
+    (y * y).sum().backward()
+    live = A._mask.cpu().numpy().astype(bool)
+    grad = A.values.grad.cpu().numpy()
+    assert np.all(grad[~live] == 0)
+    gy = 2 * (D @ x0)
+    assert pyoracle.rel_l2(A.to_input(A.values.grad).cpu().numpy(), gy[r] * x0[c]) <= 1e-12
+    assert pyoracle.rel_l2(x.grad.cpu().numpy(), D.T @ gy) <= 1e-12
+    A.close()
