@@ -56,7 +56,7 @@ class Opts(ctypes.Structure):
         ("stream_tile_kib", ctypes.c_int),
         ("backward_mode", ctypes.c_int),
         ("stream_panels", ctypes.c_int),
-        ("reserved", ctypes.c_int * 1),
+        ("stream_tiles", ctypes.c_int),
     ]
 
 

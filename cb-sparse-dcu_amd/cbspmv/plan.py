@@ -49,7 +49,7 @@ _SCHEDULES = {"auto": _lib.SCHED_AUTO, "blocked": _lib.SCHED_BLOCKED, "stream": 
 
 def make_opts(block_size=16, waves=4, row_owned=True, pattern_wave=True, force_coo64=False,
               dense_sixteenths=15, builder="gather", dtype="f64", compensated=False,
-              schedule="auto", item_nnz=0, tile_waves=0, tile_kib=0, backward="direct", panels=0):
+              schedule="auto", item_nnz=0, tile_waves=0, tile_kib=0, backward="direct", panels=0, tiles=0):
     o = _lib.Opts()
     lib().cbspmv_default_opts(ctypes.byref(o))
     o.block_size = int(block_size)
@@ -67,6 +67,7 @@ def make_opts(block_size=16, waves=4, row_owned=True, pattern_wave=True, force_c
     o.stream_tile_kib = int(tile_kib)
     o.backward_mode = {"fused": _lib.BACKWARD_FUSED, "direct": _lib.BACKWARD_DIRECT}[backward]
     o.stream_panels = int(panels)
+    o.stream_tiles = int(tiles)
     return o
 
 

@@ -24,7 +24,6 @@
 
 namespace cbs {
 constexpr int kBatchMaxRhs = 16;
-constexpr int kMI355XCUs = 256;
 // autotuner: copies of a candidate per timed batch (work units to fill)
 constexpr uint32_t kAutotuneUnits = 2048;
 constexpr uint32_t kAutotuneMaxCopies = 64;
@@ -36,6 +35,7 @@ struct cbspmv_batch {
     int dtype = CBSPMV_F64;
     bool comp = false;
     uint32_t units = 0;
+    int xcds = 8;     // XCDs of the device the batch was created on (queues)
     int threads = 64;
     size_t lds = 0;
     int grid = 8;
@@ -94,7 +94,8 @@ double unit_cost(uint32_t entries, int nrhs) {
     return kUnitFixed + kMultiWeight * entries * (double)(((uint32_t)nrhs + (uint32_t)kRhsGroup - 1) / (uint32_t)kRhsGroup);
 }
 
-// Contiguous unit ranges of (about) equal estimated cost, one per XCD.
+// Contiguous unit ranges of (about) equal estimated cost, one per XCD of the
+// current device.
 XcdSplit split_units(const cbspmv_batch& b, const int* nrhs) {
     std::vector<double> cost;
     cost.reserve(b.units);
@@ -106,13 +107,15 @@ XcdSplit split_units(const cbspmv_batch& b, const int* nrhs) {
     double total = 0.0;
     for (double c : cost) total += c;
     XcdSplit sp{};
+    const uint32_t nq = (uint32_t)b.xcds;
+    sp.nq = nq;
     double acc = 0.0;
     uint32_t u = 0, k = 1;
-    for (; u < (uint32_t)cost.size() && k < (uint32_t)kXcds; ++u) {
+    for (; u < (uint32_t)cost.size() && k < nq; ++u) {
         acc += cost[u];
-        while (k < (uint32_t)kXcds && acc >= total * k / kXcds) sp.begin[k++] = u + 1;
+        while (k < nq && acc >= total * k / nq) sp.begin[k++] = u + 1;
     }
-    for (; k <= (uint32_t)kXcds; ++k) sp.begin[k] = (uint32_t)cost.size();
+    for (; k <= nq; ++k) sp.begin[k] = (uint32_t)cost.size();
     return sp;
 }
 
@@ -298,10 +301,12 @@ int cbspmv_batch_create(cbspmv_plan* const* plans, int count, cbspmv_batch** bat
         const int by_lds = b->lds ? (int)std::max<size_t>(1, (size_t)(160 * 1024) / b->lds) : 8;
         per_cu = std::max(1, std::min(by_lds, 32 / max_items));
     }
-    const uint32_t cap = (uint32_t)(cbs::kMI355XCUs * per_cu);
+    const cbs::DeviceShape shape = cbs::device_shape();
+    b->xcds = shape.xcds;
+    const uint32_t cap = (uint32_t)(shape.cus * per_cu);
     b->grid = (int)((std::min<uint32_t>(std::max<uint32_t>(b->units, 1u), cap) + 7u) / 8u * 8u);
     int rc;
-    if ((rc = cbs::upload_bytes(b->queue, nullptr, cbs::kXcds * sizeof(uint32_t), s)) ||
+    if ((rc = cbs::upload_bytes(b->queue, nullptr, cbs::kMaxXcds * sizeof(uint32_t), s)) ||
         (rc = cbs::upload_bytes(b->partial, nullptr, std::max<size_t>(partial_elems, 1) * eb, s)) ||
         (rc = cbs::upload_bytes(b->counters, nullptr, std::max<size_t>(counter_elems, 1) * 4, s))) {
         cbspmv_batch_destroy(b);

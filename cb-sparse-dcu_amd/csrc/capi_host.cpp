@@ -38,6 +38,7 @@ int build_plan_object(const cbspmv_opts& o, int m, int n, int nnz, const int* ro
     if (o.stream_panels != 0 && o.stream_panels != 1 && o.stream_panels != 2 && o.stream_panels != 4 &&
         o.stream_panels != 8)
         return fail(CBSPMV_ERR_ARG, "stream_panels must be 0, 1, 2, 4 or 8");
+    if (o.stream_tiles < 0 || o.stream_tiles > 65536) return fail(CBSPMV_ERR_ARG, "stream_tiles must be in 0..65536");
     // stream keys hold column + row-in-strip + 2 flag bits in 32 bits
     if ((long long)n > (1ll << (30 - coord_bits(bs))))
         return fail(CBSPMV_ERR_UNSUPPORTED, "n above 2^(30 - log2 block_size) is not supported");
@@ -58,6 +59,7 @@ int build_plan_object(const cbspmv_opts& o, int m, int n, int nnz, const int* ro
     h.o.schedule = canonical ? o.schedule : CBSPMV_SCHED_STREAM;
     h.o.item_nnz = o.stream_item_nnz;
     h.o.panels = o.stream_panels;
+    h.o.tiles = o.stream_tiles > 0 ? o.stream_tiles : device_shape().cus;
     if (o.stream_tile_waves) h.o.tile_items = o.stream_tile_waves;
     else if (h.o.panels <= 1) h.o.tile_items = kWideTileItemsOpt;
     if (o.stream_tile_kib) h.o.tile_bytes = o.stream_tile_kib * 1024;

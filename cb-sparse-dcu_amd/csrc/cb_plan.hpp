@@ -22,6 +22,7 @@ enum : uint8_t { kCOO = 0, kCSR = 1, kDENSE = 3, kTypeMask = 0x7f, kUniform = 0x
 constexpr int kDefaultTileItemsOpt = 8;
 constexpr int kWideTileItemsOpt = 16;
 constexpr int kDefaultTileBytesOpt = 96 * 1024;
+constexpr int kStreamCUs = 256;         // MI355X compute units: the tile count without a device
 
 struct Options {
     int bs = 16;
@@ -38,6 +39,7 @@ struct Options {
     int tile_items = kDefaultTileItemsOpt;
     int tile_bytes = kDefaultTileBytesOpt;
     int panels = 0;          // column panels of the stream schedule (0 = by x size)
+    int tiles = kStreamCUs;  // stream tiles (one round of workgroups)
 };
 
 inline int coord_bits(int bs) { int b = 0; while ((1 << b) < bs) ++b; return b; }
@@ -77,7 +79,6 @@ constexpr int kMaxTileItems = 16;
 // LDS x cache per workgroup at most: the 160 KB of a CU less room for the
 // kernels' static LDS
 constexpr int kMaxTileBytes = 156 * 1024;
-constexpr int kStreamCUs = 256;         // MI355X compute units (default tile count)
 // LDS x-cache elements per workgroup; the last slot always holds 0
 inline int tile_slots(int bytes, int dtype) { return bytes / (dtype == CBSPMV_F64 ? 8 : 4); }
 inline int tile_lines(int bytes, int dtype) { return tile_slots(bytes, dtype) / kLineElems - 1; }
@@ -96,7 +97,12 @@ inline int stream_val_bytes(int dtype) { return dtype == CBSPMV_F64 ? 8 : dtype 
 // from a w-column slice of x that stays in its L2.  A panel item writes its
 // rows' partial sums to row r of panel q's output vector; a second launch
 // adds them to y.
-constexpr int kXcds = 8;                        // MI355X XCDs (workgroups dealt round-robin)
+// MI355X XCDs.  Column panels deal their items by workgroup index with this
+// period (workgroup b on XCD b % 8 is what makes a panel's x slice stay in
+// one L2; another placement only costs locality); the batched launch reads
+// the XCD each workgroup runs on from the hardware instead.
+constexpr int kXcds = 8;
+constexpr int kMaxXcds = 16;                    // per-XCD queues of the batched launch at most
 constexpr int64_t kPanelL2Bytes = 4ll << 20;    // x up to this size is gathered whole
 constexpr int64_t kPanelBytes = 2ll << 20;      // panel width target (bytes of x)
 // stream_panels = 0 picks panels by the size of x only when this is set

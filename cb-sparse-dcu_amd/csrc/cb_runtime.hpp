@@ -60,6 +60,13 @@ struct cbspmv_plan {
 };
 
 namespace cbs {
+// Compute units and XCDs of the current device (cached per device); MI355X's
+// 256 and 8 when no device is visible (a plan built on a host without one).
+struct DeviceShape {
+    int cus;
+    int xcds;
+};
+DeviceShape device_shape();
 int choose_schedule(const cbspmv_plan& p);
 long long stream_bytes_per_spmv(const cbspmv_plan& p);
 // Builds a plan object (builder, optional canonical slab, stream slab).

@@ -13,6 +13,14 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
+// The XCD (XCC) this wave runs on, from the hardware register XCC_ID
+// (bits 3:0; gfx940 and later).
+__device__ __forceinline__ uint32_t xcc_id() {
+    uint32_t v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return v & 0xfu;
+}
+
 // Lanes 0..lane inclusive.
 __device__ __forceinline__ uint64_t lanes_le(int lane) {
     return lane == 63 ? ~0ull : ((2ull << lane) - 1);

@@ -314,7 +314,7 @@ void layout_tiles(Plan& p, const StreamShape& sh, const Entries& E, std::vector<
         // the cut fits, then bisect back towards the smallest that fits, so
         // that few CUs are left without a tile (30M entries in FP64: 256
         // tiles instead of 249)
-        const int64_t max_items = (int64_t)kStreamCUs * p.o.tile_items;
+        const int64_t max_items = (int64_t)p.o.tiles * p.o.tile_items;
         int64_t lo = std::max<int64_t>(1024, total / max_items + 1), hi = lo;
         cut(hi);
         bool fits = (int64_t)pieces.size() <= max_items;

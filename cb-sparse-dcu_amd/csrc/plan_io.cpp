@@ -7,8 +7,15 @@
 // version + u32 byte order mark; the build options; the plan's scalars; then
 // every array as (u64 element count, u64 element size, bytes).  The reader
 // checks magic, version, element sizes and lengths, and fails with
-// CBSPMV_ERR_ARG on any mismatch or truncation.  The transposed plan is not
-// saved (cbspmv_build_transpose rebuilds it).
+// CBSPMV_ERR_ARG on any mismatch or truncation.  It then re-checks what the
+// kernels index without bounds checks, so that a damaged file is refused
+// rather than read or written out of bounds on the device: the options,
+// every stream item's entry range and base row against the slab and m,
+// every key's column (or LDS slot) and row, the tiles' line ranges against
+// the line table and the LDS cache, the cut rows' pieces against the items
+// and the partial buffer, the empty rows and the panel tables.  The
+// transposed plan is not saved (cbspmv_build_transpose rebuilds it).
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -23,7 +30,7 @@ namespace cbs {
 namespace {
 
 constexpr char kMagic[16] = "CBSPMV-PLAN";
-constexpr uint32_t kVersion = 2;
+constexpr uint32_t kVersion = 3;
 constexpr uint32_t kBom = 0x01020304u;
 
 struct Writer {
@@ -113,6 +120,79 @@ void read_offsets(Reader& r, Plan& p) {
     }
 }
 
+// What the stream kernels index without bounds checks, re-checked on load.
+bool validate(const Plan& h, std::string& why) {
+    auto bad = [&why](const char* w) { why = w; return false; };
+    const Options& o = h.o;
+    if (o.dtype < CBSPMV_F64 || o.dtype > CBSPMV_BF16 || (o.accum != CBSPMV_ACC_NATIVE && o.accum != CBSPMV_ACC_COMPENSATED))
+        return bad("value type or accumulation");
+    if (o.bs != 2 && o.bs != 4 && o.bs != 8 && o.bs != 16 && o.bs != 32 && o.bs != 64) return bad("block size");
+    if (o.tile_items < 1 || o.tile_items > kMaxTileItems || o.tile_bytes < 4096 || o.tile_bytes > kMaxTileBytes)
+        return bad("stream tile shape");
+    if (h.panels != 1 && h.panels != 2 && h.panels != 4 && h.panels != 8) return bad("panel count");
+    if (h.panels > 1 && o.tile_items > kMaxTileItems / 2) return bad("panels with more than 8 tile waves");
+    if (h.col_bits < 1 || h.col_bits > 30) return bad("column bits");
+    if (h.m < 0 || h.n < 0 || (int64_t)h.n > (1ll << h.col_bits)) return bad("matrix shape");
+    const uint32_t m = (uint32_t)h.m, n = (uint32_t)h.n;
+    const int epl = stream_lane_entries(o.dtype);
+    const size_t eb = 4 + (size_t)stream_val_bytes(o.dtype);
+    const uint32_t slots = (uint32_t)tile_slots(o.tile_bytes, o.dtype);
+    const uint32_t cmask = (1u << h.col_bits) - 1u, omask = (1u << (31 - h.col_bits)) - 1u;
+    const size_t NI = h.items.size();
+    if (h.tile_items_count > NI || h.item_split.size() != NI || h.item_split_first.size() != NI)
+        return bad("item tables");
+    if (h.tiles.size() % 2 || (uint64_t)(h.tiles.size() / 2) * (uint64_t)o.tile_items < h.tile_items_count)
+        return bad("tile table");
+    for (size_t t = 0; t < h.tiles.size() / 2; ++t) {
+        const uint64_t first = h.tiles[2 * t], cnt = h.tiles[2 * t + 1];
+        if (first + cnt > h.tile_lines.size() || cnt * kLineElems > (uint64_t)slots - 1u) return bad("tile lines");
+    }
+    for (size_t i = 0; i < NI; ++i) {
+        const StreamItem& it = h.items[i];
+        if (it.e_begin > it.e_end || it.e_begin % epl || it.e_end % epl || (size_t)it.e_end * eb > h.s_slab.size())
+            return bad("item entry range");
+        if (it.e_end > it.e_begin && it.row_base >= m) return bad("item base row");
+        if (it.first_partial > 1 || it.last_partial > 1) return bad("item flags");
+        if ((it.first_partial && h.item_split_first[i] >= h.splits.size()) ||
+            (it.last_partial && h.item_split[i] >= h.splits.size()))
+            return bad("item split ids");
+        const uint32_t chunk = (uint32_t)(64 * epl);
+        for (uint32_t e = it.e_begin; e < it.e_end; ++e) {
+            const uint32_t e0 = it.e_begin + (e - it.e_begin) / chunk * chunk;
+            const uint32_t G = std::min<uint32_t>(64u, (it.e_end - e0) / (uint32_t)epl);
+            const uint32_t l = (e - e0) / (uint32_t)epl, j = (e - e0) % (uint32_t)epl;
+            const size_t at = (size_t)e0 * eb + (size_t)(j * 4 / 16) * G * 16 + (size_t)l * 16 + (j * 4) % 16;
+            uint32_t k;
+            std::memcpy(&k, h.s_slab.data() + at, 4);
+            const uint32_t low = k & cmask;
+            if (k & kKeyCached) {
+                // panel items read no LDS (such keys are padding there)
+                if (i < h.tile_items_count && low >= slots) return bad("key LDS slot");
+            } else if (low >= n) {
+                return bad("key column");
+            }
+            if ((uint64_t)it.row_base + ((k >> h.col_bits) & omask) >= m) return bad("key row");
+        }
+    }
+    for (const StreamSplit& sp : h.splits) {
+        if (sp.row >= m || sp.pieces < 1 || (uint64_t)sp.first_item + sp.pieces > NI ||
+            (uint64_t)sp.partial_base + sp.pieces > h.partial_slots)
+            return bad("cut rows");
+    }
+    for (uint32_t r : h.empty_rows)
+        if (r >= m) return bad("empty rows");
+    if (h.panels > 1) {
+        if (h.panel_item_ptr.size() != (size_t)h.panels + 1 || h.panel_item_ptr[0] != h.tile_items_count ||
+            h.panel_item_ptr.back() > NI || h.panel_mask.size() != (size_t)m)
+            return bad("panel tables");
+        for (size_t q = 0; q + 1 < h.panel_item_ptr.size(); ++q)
+            if (h.panel_item_ptr[q] > h.panel_item_ptr[q + 1]) return bad("panel tables");
+    } else if (h.tile_items_count != NI) {
+        return bad("items outside the tiles");
+    }
+    return true;
+}
+
 }  // namespace
 }  // namespace cbs
 
@@ -167,6 +247,8 @@ int cbspmv_load(const char* path, cbspmv_plan** plan) {
     const cbs::Plan& h = p->host;
     if (h.m < 0 || h.n < 0 || h.nnz < 0 || (int64_t)h.in_row.size() != h.nnz || !h.stream_ready)
         return fail(CBSPMV_ERR_ARG, std::string("inconsistent plan file: ") + path);
+    std::string why;
+    if (!cbs::validate(h, why)) return fail(CBSPMV_ERR_ARG, std::string("invalid plan file (") + why + "): " + path);
     p->schedule = cbs::choose_schedule(*p);
     *plan = p.release();
     return CBSPMV_OK;

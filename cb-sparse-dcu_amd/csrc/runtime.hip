@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -35,6 +36,26 @@ int upload_bytes(DeviceBuf& b, const void* src, size_t bytes, hipStream_t s) {
 }
 
 size_t vec_elem_bytes(const Plan& h) { return h.o.dtype == CBSPMV_F64 ? 8 : 4; }
+
+DeviceShape device_shape() {
+    static std::mutex mu;
+    static std::vector<DeviceShape> known;
+    DeviceShape def{kStreamCUs, kXcds};
+    int count = 0, dev = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return def;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if ((size_t)dev >= known.size()) known.resize((size_t)dev + 1, DeviceShape{0, 0});
+    if (known[dev].cus == 0) {
+        int cus = 0, xcds = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = def.cus;
+        if (hipDeviceGetAttribute(&xcds, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess || xcds <= 0) xcds = 1;
+        known[dev] = DeviceShape{cus, std::min(xcds, kMaxXcds)};
+    }
+    return known[dev];
+}
 
 void release_device(DevicePlan& d) {
     for (DeviceBuf* b : {&d.slab, &d.zero_strips, &d.key, &d.val, &d.items, &d.item_split,

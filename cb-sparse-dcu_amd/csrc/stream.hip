@@ -867,7 +867,8 @@ __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ uint32_t next_unit;
     A* xs = reinterpret_cast<A*>(lds_raw);
-    uint32_t q = blockIdx.x & 7u;   // the hardware deals workgroups round-robin over the 8 XCDs
+    const uint32_t nq = split.nq;
+    uint32_t q = xcc_id() % nq;   // start with this XCD's own queue
     uint32_t queues_done = 0;
     for (;;) {
         if (threadIdx.x == 0) next_unit = split.begin[q] + atomicAdd(queue + q, 1u);
@@ -875,8 +876,8 @@ __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan
         const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)next_unit);
         __syncthreads();   // next_unit is rewritten at the next unit
         if (u >= split.begin[q + 1]) {
-            if (++queues_done == (uint32_t)kXcds) break;
-            q = (q + 1u) & 7u;
+            if (++queues_done == nq) break;
+            q = q + 1u == nq ? 0u : q + 1u;
             continue;
         }
         // plan of unit u: last plan with unit_begin <= u (binary search)
@@ -948,7 +949,7 @@ __device__ __forceinline__ void store_rhs(double* p, const double (&v)[R], uint3
 template <typename V, typename A>
 __global__ __launch_bounds__(256) void batch_prologue(const BatchPlan<V, A>* plans, const uint32_t* block_plan,
                                                       uint32_t nblocks, uint32_t* queue) {
-    if (blockIdx.x == 0 && threadIdx.x < kXcds) queue[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kMaxXcds) queue[threadIdx.x] = 0;
     if (blockIdx.x >= nblocks) return;   // the one block launched when no plan needs x interleaved
     const BatchPlan<V, A>& bp = plans[block_plan[blockIdx.x]];
     const uint32_t n = (uint32_t)bp.args.n;

@@ -73,13 +73,14 @@ struct BatchPlan {
 // The work units (plan, tile) of XCD k in a batched launch:
 // [begin[k], begin[k + 1]).
 struct XcdSplit {
-    uint32_t begin[kXcds + 1];
+    uint32_t nq;                   // queues: the device's XCD count (1..kMaxXcds)
+    uint32_t begin[kMaxXcds + 1];
 };
 
 // Interleaved columns (groups * n) per block of batch_prologue.
 constexpr uint32_t kPrologueCols = 1024;
 
-// Runs before every launch_batch, on the same stream: zeroes the kXcds unit
+// Runs before every launch_batch, on the same stream: zeroes the kMaxXcds unit
 // queue counters and fills BatchPlan::xg of every plan with nrhs > 1 from
 // its x.  block_plan[b] (device memory, nblocks entries, 0 if no plan has
 // several right-hand sides) is the plan of block b; plan p's blocks start

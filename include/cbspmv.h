@@ -80,7 +80,9 @@ typedef struct cbspmv_opts {
     int stream_tile_kib;  /* stream schedule: LDS x cache per workgroup, 4..156 KiB (0=96) */
     int backward_mode;    /* enum cbspmv_backward_mode (cbspmv_backward)              */
     int stream_panels;    /* stream schedule: x column panels 1, 2, 4 or 8 (0 = 1)    */
-    int reserved[1];
+    int stream_tiles;     /* stream schedule: tiles (workgroups of one round), 0 = the
+                             current device's compute units (256, MI355X's count, when
+                             no device is visible at build time); 1..65536            */
 } cbspmv_opts;
 
 /* Builder statistics: the GLOBAL_BLOCK scalars (format.h:63-77) plus counts. */

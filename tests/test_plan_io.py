@@ -76,3 +76,70 @@ def test_loaded_plan_spmv(tmp_path, gpu):
     assert torch.equal(p.spmv(x), q.spmv(x))   # same slab, same bits
     q.close()
     p.close()
+
+
+def _corrupt(src, dst, find, patch_at, patch):
+    """Copies plan file src to dst with `patch` written over the bytes that
+    start `patch_at` bytes into the first occurrence of `find`."""
+    raw = bytearray(open(src, "rb").read())
+    at = bytes(raw).find(find)
+    assert at >= 0, "array not found in the file"
+    raw[at + patch_at:at + patch_at + len(patch)] = patch
+    open(dst, "wb").write(bytes(raw))
+
+
+@pytest.mark.parametrize("what", ["item_end", "item_row", "key_column", "key_row", "tile_lines", "split_piece"])
+def test_damaged_index_arrays_are_refused(tmp_path, what):
+    """A file whose lengths are consistent but whose indices point outside
+    the arrays the kernels index is refused on load (CBSPMV_ERR_ARG), not
+    uploaded: an item past the slab or past m, a key naming a column past
+    n or a row past m, a tile with more lines than its table or its LDS
+    cache, a cut row's pieces past the partial buffer."""
+    m, n = 500, 800
+    r, c, v = gen.random_coo(m, n, 9, seed=8)
+    # row 250 takes every column: longer than four items, so it is cut into pieces
+    keep = r != 250
+    at = int(np.searchsorted(r, 250))
+    r = np.concatenate([r[:at], np.full(n, 250, np.int32), r[keep][at:]])
+    c = np.concatenate([c[:at], np.arange(n, dtype=np.int32), c[keep][at:]])
+    v = np.ones(r.size)
+    p = Plan(m, n, r, c, v, item_nnz=64, tile_waves=4)
+    good = tmp_path / "good.cbplan"
+    p.save(good)
+    Plan.load(good).close()
+    items = p.export("stream_items").astype(np.uint32)
+    slab = p.export("stream_slab")
+    bad = tmp_path / "bad.cbplan"
+    cb = int(p.export("stream_col_bits")[0])
+    if what == "item_end":
+        _corrupt(good, bad, items.tobytes(), 4, np.uint32(items[1] + (1 << 20)).tobytes())
+    elif what == "item_row":
+        _corrupt(good, bad, items.tobytes(), 8, np.uint32(m + 7).tobytes())
+    elif what in ("key_column", "key_row"):
+        key = np.frombuffer(slab[:4].tobytes(), np.uint32)[0]
+        if what == "key_column":
+            new = np.uint32(n + 3)                                   # uncached, column past n
+        else:
+            new = np.uint32((key & ((1 << cb) - 1)) | ((((1 << (31 - cb)) - 1) << cb) & 0x7FFFFFFF))
+        _corrupt(good, bad, slab[:4096].tobytes(), 0, new.tobytes())
+    elif what == "tile_lines":
+        tiles = p.export("stream_tiles").astype(np.uint32)
+        _corrupt(good, bad, tiles.tobytes(), 4, np.uint32(1 << 20).tobytes())
+    else:
+        assert p.stats()["nnz"] and items.size
+        splits_items = items.reshape(-1, 4)[:, 3]
+        assert np.any(splits_items != 0), "fixture must cut a row into pieces"
+        # splits are (row, first_item, pieces, partial_base) u32 records: find the row-250 record
+        raw = open(good, "rb").read()
+        want = np.uint32(250).tobytes()
+        # the split record starts with the row; pieces two words later
+        pos = [i for i in range(0, len(raw) - 16, 4) if raw[i:i + 4] == want
+               and np.frombuffer(raw[i + 8:i + 12], np.uint32)[0] >= 2
+               and np.frombuffer(raw[i + 8:i + 12], np.uint32)[0] < 64]
+        assert pos
+        b = bytearray(raw)
+        b[pos[0] + 12:pos[0] + 16] = np.uint32(1 << 24).tobytes()   # partial_base far past the buffer
+        open(bad, "wb").write(bytes(b))
+    with pytest.raises(CbspmvError, match="invalid plan file"):
+        Plan.load(bad)
+    p.close()
