@@ -83,15 +83,27 @@ void fill_args(const cbspmv_plan& p, StreamArgs<V, A>& a) {
     a.n = h.n;
 }
 
+// Group width of a plan with several right-hand sides: 4 when they fit one
+// group of 4 (16 bytes per FP32 column, so half the cache lines a group of
+// 8 would touch), else 8.  Splitting 5-8 right-hand sides of the largest
+// plans into two groups of 4, so that one group's interleaved x stays in
+// an XCD's L2, measured slower on the 256-matrix batch (2.13 against
+// 2.05 ms): the second pass over their slab costs more than the gathers
+// it takes off the fabric.
+uint32_t group_width(const Plan& h, int nrhs) {
+    (void)h;
+    return nrhs <= kRhsGroupNarrow ? (uint32_t)kRhsGroupNarrow : (uint32_t)kRhsGroup;
+}
+
 // Estimated cost of one work unit, in stream entries: a tile with one
 // right-hand side streams its entries once with its LDS x cache; with
-// several, once per group of kRhsGroup, gathering every entry's x values
-// (about 1.5x the one-RHS cost per entry); plus a fixed cost per unit.
-double unit_cost(uint32_t entries, int nrhs) {
+// several, once per group of gw, gathering every entry's x values (about
+// 1.5x the one-RHS cost per entry); plus a fixed cost per unit.
+double unit_cost(uint32_t entries, int nrhs, uint32_t gw) {
     constexpr double kUnitFixed = 2048.0, kMultiWeight = 1.5;
     if (nrhs <= 0) return kUnitFixed;
     if (nrhs == 1) return kUnitFixed + entries;
-    return kUnitFixed + kMultiWeight * entries * (double)(((uint32_t)nrhs + (uint32_t)kRhsGroup - 1) / (uint32_t)kRhsGroup);
+    return kUnitFixed + kMultiWeight * entries * (double)(((uint32_t)nrhs + gw - 1) / gw);
 }
 
 // Contiguous unit ranges of (about) equal estimated cost, one per XCD of the
@@ -101,8 +113,9 @@ XcdSplit split_units(const cbspmv_batch& b, const int* nrhs) {
     cost.reserve(b.units);
     for (size_t i = 0; i < b.plans.size(); ++i) {
         const std::vector<uint32_t>& te = b.tile_entries[i];
-        if (te.empty()) cost.push_back(unit_cost(0, nrhs[i]));
-        for (uint32_t e : te) cost.push_back(unit_cost(e, nrhs[i]));
+        const uint32_t gw = group_width(b.plans[i]->host, nrhs[i]);
+        if (te.empty()) cost.push_back(unit_cost(0, nrhs[i], gw));
+        for (uint32_t e : te) cost.push_back(unit_cost(e, nrhs[i], gw));
     }
     double total = 0.0;
     for (double c : cost) total += c;
@@ -134,13 +147,14 @@ int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* 
         if (nrhs[i] < 0 || nrhs[i] > kBatchMaxRhs) return fail(CBSPMV_ERR_ARG, "nrhs must be in 0..16");
         if (nrhs[i] > 0 && ((h.m > 0 && !y[i]) || (h.nnz > 0 && !x[i]))) return fail(CBSPMV_ERR_ARG, "null vector");
         if (nrhs[i] > 1) {
-            const uint64_t cols = ((uint64_t)nrhs[i] + (uint32_t)kRhsGroup - 1) / (uint32_t)kRhsGroup * (uint64_t)h.n;
-            xg_elems += (size_t)cols * (uint32_t)kRhsGroup;
+            const uint32_t gw = group_width(h, nrhs[i]);
+            const uint64_t cols = ((uint64_t)nrhs[i] + gw - 1) / gw * (uint64_t)h.n;
+            xg_elems += (size_t)cols * gw;
             cols_sum += cols;
         }
     }
     if (cols_sum >= (1ull << 31))
-        return fail(CBSPMV_ERR_UNSUPPORTED, "batch: sum over plans of n x ceil(nrhs / 8) must stay below 2^31");
+        return fail(CBSPMV_ERR_UNSUPPORTED, "batch: sum over plans of n x (right-hand-side groups) must stay below 2^31");
     std::vector<uint32_t> block_plan;
     if (xg.bytes < xg_elems * sizeof(A)) {
         const int rc = upload_bytes(xg, nullptr, xg_elems * sizeof(A), s);
@@ -169,11 +183,12 @@ int make_desc(cbspmv_batch& b, const void* const* x, void* const* y, const int* 
         e.partial = part;
         e.counters = cnt;
         e.xg = nullptr;
+        e.gw = group_width(h, nrhs[i]);
         e.xg_block0 = (uint32_t)block_plan.size();
         if (nrhs[i] > 1) {
-            const size_t cols = (size_t)((uint32_t)nrhs[i] + (uint32_t)kRhsGroup - 1) / (uint32_t)kRhsGroup * (size_t)h.n;
+            const size_t cols = (size_t)((uint32_t)nrhs[i] + e.gw - 1) / e.gw * (size_t)h.n;
             e.xg = static_cast<A*>(xg.ptr) + xg_off;
-            xg_off += cols * (uint32_t)kRhsGroup;
+            xg_off += cols * e.gw;
             block_plan.insert(block_plan.end(), (cols + kPrologueCols - 1) / kPrologueCols, (uint32_t)i);
         }
         part += (size_t)h.partial_slots * parts * kBatchMaxRhs;

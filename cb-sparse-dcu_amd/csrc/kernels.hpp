@@ -26,6 +26,19 @@ __device__ __forceinline__ uint64_t lanes_le(int lane) {
     return lane == 63 ? ~0ull : ((2ull << lane) - 1);
 }
 
+// ---- global-memory pointers ----------------------------------------------------
+// A pointer the compiler cannot prove global (one read from a descriptor in
+// device memory) makes its loads and stores flat instructions, and a flat
+// load counts against the LDS wait counter too, so every wait for an LDS
+// read would also wait for the gathers in flight.  Every vector this code
+// addresses is device memory: gptr() states that.
+template <typename T>
+using gbl = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ const gbl<T>* gptr(const T* p) { return (const gbl<T>*)p; }
+template <typename T>
+__device__ __forceinline__ gbl<T>* gptr(T* p) { return (gbl<T>*)p; }
+
 // ---- value types ------------------------------------------------------------
 // BF16 values are stored as the upper half of an FP32 bit pattern.
 struct bf16_t { uint16_t bits; };

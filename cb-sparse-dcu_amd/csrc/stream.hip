@@ -39,6 +39,19 @@ __device__ __forceinline__ A load_partial(const A* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A stream item's 16-byte record, read as one global load.
+__device__ __forceinline__ StreamItem load_item(const StreamItem* items, uint32_t i) {
+    typedef uint32_t w4 __attribute__((ext_vector_type(4)));
+    const w4 w = gptr(reinterpret_cast<const w4*>(items))[i];
+    return __builtin_bit_cast(StreamItem, w);
+}
+
+__device__ __forceinline__ StreamSplit load_split(const StreamSplit* splits, uint32_t i) {
+    typedef uint32_t w4 __attribute__((ext_vector_type(4)));
+    const w4 w = gptr(reinterpret_cast<const w4*>(splits))[i];
+    return __builtin_bit_cast(StreamSplit, w);
+}
+
 // (hi, lo) pairs for the compensated accumulation
 __device__ __forceinline__ f2& operator+=(f2& a, const f2& b) {
     a = add_f2(a, b);
@@ -95,12 +108,12 @@ struct Chunk {
         const uint32_t l = min((uint32_t)lane, G - 1) * 16;
 #pragma unroll
         for (int p = 0; p < NK; ++p) {
-            const u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(base + p * G * 16 + l));
+            const u32x4 w = __builtin_nontemporal_load(gptr(reinterpret_cast<const u32x4*>(base + p * G * 16 + l)));
             k[4 * p] = w.x; k[4 * p + 1] = w.y; k[4 * p + 2] = w.z; k[4 * p + 3] = w.w;
         }
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
-            const u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(base + (NK + q) * G * 16 + l));
+            const u32x4 w = __builtin_nontemporal_load(gptr(reinterpret_cast<const u32x4*>(base + (NK + q) * G * 16 + l)));
             uint32_t ws[4] = {w.x, w.y, w.z, w.w};
             __builtin_memcpy(&v[q * (16 / sizeof(V))], ws, 16);
         }
@@ -192,14 +205,14 @@ __device__ __forceinline__ void stage_tile(const StreamArgs<V, A>& a, uint32_t t
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const uint32_t s = s0 + j * nthreads;
-            lid[j] = s < cnt ? a.tile_lines[lb + s / kLineElems] : 0u;
+            lid[j] = s < cnt ? gptr(a.tile_lines)[lb + s / kLineElems] : 0u;
         }
         A xv[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const uint32_t s = s0 + j * nthreads;
             const uint32_t c = lid[j] * kLineElems + (s % kLineElems);
-            xv[j] = (s < cnt && c < (uint32_t)a.n) ? a.x[c] : A(0);
+            xv[j] = (s < cnt && c < (uint32_t)a.n) ? gptr(a.x)[c] : A(0);
         }
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
@@ -226,7 +239,7 @@ __device__ __forceinline__ void join_pieces(const StreamArgs<V, A>& a, uint32_t 
         uint32_t ticket = 0;
         if (lane == 0) ticket = atomicAdd(a.counters + sid, 1u);
         ticket = __shfl(ticket, 0, kWave);
-        const StreamSplit sp = a.splits[sid];
+        const StreamSplit sp = load_split(a.splits, sid);
         if (ticket == sp.pieces - 1) {
             __threadfence();
             if (lane == 0) {
@@ -255,7 +268,7 @@ __device__ __forceinline__ void row_products(const A* xrow, uint32_t m, uint32_t
                                              bool valid, const A (&xc)[EPL], A (&dv)[EPL]) {
     const int lane = lane_id();
     const uint32_t r0 = row_base + (uint32_t)__builtin_amdgcn_readfirstlane(off[0]);
-    const A xw = xrow[min(r0 + (uint32_t)lane, m - 1u)];
+    const A xw = gptr(xrow)[min(r0 + (uint32_t)lane, m - 1u)];
     const uint32_t dlo = row_base + (uint32_t)off[0] - r0, dhi = row_base + (uint32_t)off[EPL - 1] - r0;
     const A xa = __shfl(xw, valid && dlo < 64u ? (int)dlo : 0, kWave);
     const A xb = __shfl(xw, valid && dhi < 64u ? (int)dhi : 0, kWave);
@@ -271,7 +284,7 @@ __device__ __forceinline__ void row_products(const A* xrow, uint32_t m, uint32_t
         for (int j = 0; j < EPL; ++j) {
             const uint32_t d = row_base + (uint32_t)off[j] - r0;
             const A v = __shfl(xw, valid && d < 64u ? (int)d : 0, kWave);
-            if (slow) dv[j] = xc[j] * (d < 64u ? v : xrow[min(row_base + (uint32_t)off[j], m - 1u)]);
+            if (slow) dv[j] = xc[j] * (d < 64u ? v : gptr(xrow)[min(row_base + (uint32_t)off[j], m - 1u)]);
         }
     }
 }
@@ -293,18 +306,18 @@ __device__ __forceinline__ void stream_item_near(const StreamArgs<V, A>& a, uint
     using Acc = typename std::conditional<COMP, f2, A>::type;
     const uint32_t kZero = a.tile_slots - 1;
     const int lane = lane_id();
-    const StreamItem it = a.items[item];
+    const StreamItem it = load_item(a.items, item);
     const int cb = a.col_bits;
     const uint32_t cmask = (1u << cb) - 1u;
     const uint32_t omask = (1u << (31 - cb)) - 1u;
-    const uint32_t split_first = it.first_partial ? a.item_split_first[item] : 0xffffffffu;
-    const uint32_t split_last = it.last_partial ? a.item_split[item] : 0xffffffffu;
+    const uint32_t split_first = it.first_partial ? gptr(a.item_split_first)[item] : 0xffffffffu;
+    const uint32_t split_last = it.last_partial ? gptr(a.item_split)[item] : 0xffffffffu;
     auto put_partial = [&](uint32_t slot, Acc v) {
         if constexpr (COMP) {
-            a.partial[2 * slot] = v.hi;
-            a.partial[2 * slot + 1] = v.lo;
+            gptr(a.partial)[2 * slot] = v.hi;
+            gptr(a.partial)[2 * slot + 1] = v.lo;
         } else {
-            a.partial[slot] = v;
+            gptr(a.partial)[slot] = v;
         }
     };
     // a finished run: to y, or to the partial buffer when it is a piece of
@@ -312,13 +325,13 @@ __device__ __forceinline__ void stream_item_near(const StreamArgs<V, A>& a, uint
     auto emit = [&](bool go, int off, Acc v, bool final_run) {
         if (!go) return;
         if (it.first_partial && off == 0) {
-            const StreamSplit sp = a.splits[split_first];
+            const StreamSplit sp = load_split(a.splits, split_first);
             put_partial(sp.partial_base + (item - sp.first_item), v);
         } else if (final_run && it.last_partial) {
-            const StreamSplit sp = a.splits[split_last];
+            const StreamSplit sp = load_split(a.splits, split_last);
             put_partial(sp.partial_base + (item - sp.first_item), v);
         } else {
-            yout[it.row_base + (uint32_t)off] = finish(v);
+            gptr(yout)[it.row_base + (uint32_t)off] = finish(v);
         }
     };
     auto gather = [&](const uint32_t* kk, A* xx) {
@@ -327,7 +340,7 @@ __device__ __forceinline__ void stream_item_near(const StreamArgs<V, A>& a, uint
             const uint32_t c = kk[j] & cmask;
             const bool cached = (kk[j] & kKeyCached) != 0;
             A g = A(0);
-            if (!cached) g = a.x[c];
+            if (!cached) g = gptr(a.x)[c];
             A l = A(0);
             if constexpr (LDS) l = xs[min(c, kZero)];
             xx[j] = cached ? l : g;
@@ -359,7 +372,7 @@ __device__ __forceinline__ void stream_item_near(const StreamArgs<V, A>& a, uint
             A dv[EPL];
             row_products<A, EPL>(a.xrow, (uint32_t)a.m, it.row_base, off, valid, xc, dv);
             if (valid) {
-                u32x4* dst = reinterpret_cast<u32x4*>(a.dvals + e0 + (uint32_t)lane * EPL);
+                gbl<u32x4>* dst = gptr(reinterpret_cast<u32x4*>(a.dvals + e0 + (uint32_t)lane * EPL));
 #pragma unroll
                 for (int q = 0; q < EPL * (int)sizeof(A) / 16; ++q) {
                     u32x4 w;
@@ -421,12 +434,12 @@ __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t 
     using Acc = typename std::conditional<COMP, f2, A>::type;
     const uint32_t kZero = a.tile_slots - 1;
     const int lane = lane_id();
-    const StreamItem it = a.items[item];
+    const StreamItem it = load_item(a.items, item);
     const int cb = a.col_bits;
     const uint32_t cmask = (1u << cb) - 1u;
     const uint32_t omask = (1u << (31 - cb)) - 1u;
-    const uint32_t split_first = it.first_partial ? a.item_split_first[item] : 0xffffffffu;
-    const uint32_t split_last = it.last_partial ? a.item_split[item] : 0xffffffffu;
+    const uint32_t split_first = it.first_partial ? gptr(a.item_split_first)[item] : 0xffffffffu;
+    const uint32_t split_last = it.last_partial ? gptr(a.item_split)[item] : 0xffffffffu;
 
     // A finished run (row offset `off`) goes to y, or to the partial buffer
     // when it is a piece of a cut row (the item's first run, offset 0, or its
@@ -564,7 +577,7 @@ __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t 
         if (hm != 0) {
             const Acc v = read_lane(held, (int)__builtin_ctzll(hm));
             if (lane == 0) {
-                const StreamSplit sp = a.splits[split_first];
+                const StreamSplit sp = load_split(a.splits, split_first);
                 put_partial(sp.partial_base + (item - sp.first_item), v);
             }
         }
@@ -573,10 +586,10 @@ __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t 
     // holds one run), or a whole row
     if (lane == 0 && carry_off >= 0) {
         if (first_piece && carry_off == 0) {
-            const StreamSplit sp = a.splits[split_first];
+            const StreamSplit sp = load_split(a.splits, split_first);
             put_partial(sp.partial_base + (item - sp.first_item), carry);
         } else if (it.last_partial) {
-            const StreamSplit sp = a.splits[split_last];
+            const StreamSplit sp = load_split(a.splits, split_last);
             put_partial(sp.partial_base + (item - sp.first_item), carry);
         } else {
             yout[it.row_base + (uint32_t)carry_off] = finish(carry);
@@ -621,7 +634,7 @@ __device__ __forceinline__ void load_rhs(const float* p, float (&v)[R], uint32_t
 #pragma unroll
     for (int q = 0; q < R / 4; ++q) {
         f32x4 w = {0.f, 0.f, 0.f, 0.f};
-        if ((uint32_t)(4 * q) < nr) w = *reinterpret_cast<const f32x4*>(p + 4 * q);
+        if ((uint32_t)(4 * q) < nr) w = *gptr(reinterpret_cast<const f32x4*>(p + 4 * q));
         v[4 * q] = w.x; v[4 * q + 1] = w.y; v[4 * q + 2] = w.z; v[4 * q + 3] = w.w;
     }
 }
@@ -630,7 +643,7 @@ __device__ __forceinline__ void load_rhs(const double* p, double (&v)[R], uint32
 #pragma unroll
     for (int q = 0; q < R / 2; ++q) {
         f64x2 w = {0.0, 0.0};
-        if ((uint32_t)(2 * q) < nr) w = *reinterpret_cast<const f64x2*>(p + 2 * q);
+        if ((uint32_t)(2 * q) < nr) w = *gptr(reinterpret_cast<const f64x2*>(p + 2 * q));
         v[2 * q] = w.x; v[2 * q + 1] = w.y;
     }
 }
@@ -647,30 +660,30 @@ __device__ __forceinline__ void load_rhs(const double* p, double (&v)[R], uint32
 // stream_tile_multi deals the tile's items to the workgroup's waves.
 template <typename V, typename A, bool COMP, int EPL, int R>
 __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t item, const uint32_t* lines,
-                                           uint64_t ldy, uint64_t pstride, uint32_t nr) {
+                                           uint64_t ldy, uint64_t pstride, uint32_t nr, uint32_t gw) {
     using Acc = typename std::conditional<COMP, f2, A>::type;
     const uint32_t kZero = a.tile_slots - 1;
     const int lane = lane_id();
-    const StreamItem it = a.items[item];
+    const StreamItem it = load_item(a.items, item);
     const int cb = a.col_bits;
     const uint32_t cmask = (1u << cb) - 1u;
     const uint32_t omask = (1u << (31 - cb)) - 1u;
-    const uint32_t split_first = it.first_partial ? a.item_split_first[item] : 0xffffffffu;
-    const uint32_t split_last = it.last_partial ? a.item_split[item] : 0xffffffffu;
+    const uint32_t split_first = it.first_partial ? gptr(a.item_split_first)[item] : 0xffffffffu;
+    const uint32_t split_last = it.last_partial ? gptr(a.item_split)[item] : 0xffffffffu;
     constexpr int kParts = COMP ? 2 : 1;
 
     auto emit = [&](bool go, int off, Acc v, bool final_run, uint32_t r) {
         if (!go) return;
         uint32_t slot = 0xffffffffu;
         if (it.first_partial && off == 0) {
-            const StreamSplit sp = a.splits[split_first];
+            const StreamSplit sp = load_split(a.splits, split_first);
             slot = sp.partial_base + (item - sp.first_item);
         } else if (final_run && it.last_partial) {
-            const StreamSplit sp = a.splits[split_last];
+            const StreamSplit sp = load_split(a.splits, split_last);
             slot = sp.partial_base + (item - sp.first_item);
         }
         if (slot != 0xffffffffu) {
-            A* p = a.partial + (size_t)r * pstride;
+            gbl<A>* p = gptr(a.partial + (size_t)r * pstride);
             if constexpr (COMP) {
                 p[2 * slot] = v.hi;
                 p[2 * slot + 1] = v.lo;
@@ -678,7 +691,7 @@ __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t i
                 p[slot] = v;
             }
         } else {
-            a.y[(size_t)r * ldy + it.row_base + (uint32_t)off] = finish(v);
+            gptr(a.y)[(size_t)r * ldy + it.row_base + (uint32_t)off] = finish(v);
         }
     };
     // column of every entry: padding reads column 0 with value 0
@@ -720,7 +733,7 @@ __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t i
         const bool carried = carry_off >= 0 && o0 == carry_off;
         A xr[EPL][R];
 #pragma unroll
-        for (int j = 0; j < EPL; ++j) load_rhs<R>(a.x + (size_t)col[j] * R, xr[j], nr);
+        for (int j = 0; j < EPL; ++j) load_rhs<R>(a.x + (size_t)col[j] * gw, xr[j], nr);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if ((uint32_t)r >= nr) break;
@@ -768,7 +781,7 @@ __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t i
         uint32_t ticket = 0;
         if (lane == 0) ticket = atomicAdd(a.counters + sid, 1u);
         ticket = __shfl(ticket, 0, kWave);
-        const StreamSplit sp = a.splits[sid];
+        const StreamSplit sp = load_split(a.splits, sid);
         if (ticket == sp.pieces - 1) {
             __threadfence();
             if (lane == 0) {
@@ -788,20 +801,25 @@ __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t i
     }
 }
 
+// The workgroup's LDS holds the tile's line table (to decode the columns of
+// keys that name LDS slots); x is gathered from the interleaved copy for
+// every entry.  Staging the lines' interleaved x in LDS as well, when all
+// of a tile's lines fit, made the 256-matrix batch slower (2.22 against
+// 2.02 ms): a banded matrix's gathers already hit the vector L1, and the
+// staging delays every unit's start.
 template <typename V, typename A, bool COMP, int EPL, int R>
-__device__ __forceinline__ void stream_tile_multi(const StreamArgs<V, A>& a, uint32_t tile, uint32_t* lines,
-                                                  uint64_t ldy, uint64_t pstride, uint32_t nr) {
-    {
-        const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1];
-        for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) lines[s] = a.tile_lines[lb + s];
-    }
+__device__ __forceinline__ void stream_tile_multi(const StreamArgs<V, A>& a, uint32_t tile, unsigned char* lds,
+                                                  uint64_t ldy, uint64_t pstride, uint32_t nr, uint32_t gw) {
+    uint32_t* lines = reinterpret_cast<uint32_t*>(lds);
+    const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1];
+    for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) lines[s] = gptr(a.tile_lines)[lb + s];
     __syncthreads();
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // wave-uniform
     const uint32_t waves = blockDim.x >> 6;
     for (uint32_t w = wave; w < a.tile_items; w += waves) {
         const uint32_t item = tile * a.tile_items + w;
         if (item >= a.n_items) break;
-        multi_item<V, A, COMP, EPL, R>(a, item, lines, ldy, pstride, nr);
+        multi_item<V, A, COMP, EPL, R>(a, item, lines, ldy, pstride, nr, gw);
     }
 }
 
@@ -862,7 +880,7 @@ __global__ __launch_bounds__(256) void panel_sums(A* y, const A* out, const uint
 // unit runs its tile for every right-hand side of its plan in turn.  Unit 0
 // of a plan also zeroes the plan's empty rows.
 template <typename V, typename A, bool COMP, int EPL>
-__global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan<V, A>* plans, uint32_t nplans,
+__global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan<V, A>* __restrict__ plans, uint32_t nplans,
                                                                   XcdSplit split, uint32_t* queue) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ uint32_t next_unit;
@@ -892,7 +910,7 @@ __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan
         if (tile == 0)
             for (uint32_t r = 0; r < bp.nrhs; ++r)
                 for (uint32_t t = threadIdx.x; t < bp.args.n_empty_rows; t += blockDim.x)
-                    bp.y[(size_t)r * bp.ldy + bp.args.empty_rows[t]] = A(0);
+                    gptr(bp.y)[(size_t)r * bp.ldy + gptr(bp.args.empty_rows)[t]] = A(0);
         if (tile >= bp.args.n_tiles) continue;
         if (bp.nrhs == 1) {   // one right-hand side: the LDS x cache
             StreamArgs<V, A> a = bp.args;
@@ -904,24 +922,25 @@ __global__ __launch_bounds__(kMaxBatchThreads) void stream_batch(const BatchPlan
             __syncthreads();   // LDS is restaged by the next unit
             continue;
         }
-        // several: groups of up to kMultiRhs share each chunk load and each
-        // x gather (x interleaved per group by batch_prologue)
-        for (uint32_t r0 = 0; r0 < bp.nrhs; r0 += kMultiRhs) {
+        // several: groups of up to gw share each chunk load and each x
+        // gather (x interleaved per group by batch_prologue)
+        const uint32_t gw = bp.gw;
+        for (uint32_t r0 = 0; r0 < bp.nrhs; r0 += gw) {
             StreamArgs<V, A> a = bp.args;
             a.x = bp.xg + (size_t)r0 * bp.args.n;
             a.y = bp.y + (size_t)r0 * bp.ldy;
             a.partial = bp.partial + (size_t)r0 * bp.partial_stride;
             a.counters = bp.counters + (size_t)r0 * bp.counter_stride;
-            stream_tile_multi<V, A, COMP, EPL, kMultiRhs>(a, tile, reinterpret_cast<uint32_t*>(xs), bp.ldy,
-                                                          bp.partial_stride, min(bp.nrhs - r0, (uint32_t)kMultiRhs));
+            stream_tile_multi<V, A, COMP, EPL, kMultiRhs>(a, tile, lds_raw, bp.ldy, bp.partial_stride,
+                                                          min(bp.nrhs - r0, gw), gw);
             __syncthreads();
         }
     }
 }
 
 // batch_prologue writes the x of every plan with several right-hand sides
-// interleaved in groups of kMultiRhs for stream_tile_multi:
-// xg[(g*n + col)*kMultiRhs + j] = x[(g*kMultiRhs + j)*ldx + col].  Reads are
+// interleaved in groups of gw (the plan's group width) for stream_tile_multi:
+// xg[(g*n + col)*gw + j] = x[(g*gw + j)*ldx + col].  Reads are
 // coalesced over col; each thread writes the 16-byte vectors of one column
 // that hold a right-hand side (entries past the last one are 0; vectors
 // holding none are neither written nor loaded by load_rhs).
@@ -953,7 +972,8 @@ __global__ __launch_bounds__(256) void batch_prologue(const BatchPlan<V, A>* pla
     if (blockIdx.x >= nblocks) return;   // the one block launched when no plan needs x interleaved
     const BatchPlan<V, A>& bp = plans[block_plan[blockIdx.x]];
     const uint32_t n = (uint32_t)bp.args.n;
-    const uint32_t cols = (bp.nrhs + kMultiRhs - 1) / kMultiRhs * n;
+    const uint32_t gw = bp.gw;
+    const uint32_t cols = (bp.nrhs + gw - 1) / gw * n;
     const uint32_t c0 = (blockIdx.x - bp.xg_block0) * kPrologueCols + threadIdx.x;
 #pragma unroll
     for (uint32_t k = 0; k < kPrologueCols / 256; ++k) {
@@ -963,10 +983,10 @@ __global__ __launch_bounds__(256) void batch_prologue(const BatchPlan<V, A>* pla
         A v[kMultiRhs];
 #pragma unroll
         for (int j = 0; j < kMultiRhs; ++j) {
-            const uint32_t r = g * kMultiRhs + (uint32_t)j;
-            v[j] = r < bp.nrhs ? bp.x[(size_t)r * bp.ldx + col] : A(0);
+            const uint32_t r = g * gw + (uint32_t)j;
+            v[j] = (uint32_t)j < gw && r < bp.nrhs ? bp.x[(size_t)r * bp.ldx + col] : A(0);
         }
-        store_rhs<kMultiRhs>(bp.xg + (size_t)c * kMultiRhs, v, bp.nrhs - g * kMultiRhs);
+        store_rhs<kMultiRhs>(bp.xg + (size_t)c * gw, v, min(bp.nrhs - g * gw, gw));
     }
 }
 

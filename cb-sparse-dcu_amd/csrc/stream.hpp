@@ -15,9 +15,11 @@ constexpr int kMaxStreamThreads = 64 * kMaxTileItems;   // one wave per tile ite
 // values of eight right-hand sides for a chunk: ~210 VGPRs); the items of a
 // tile with more are dealt to its waves in turn.
 constexpr int kMaxBatchTileItems = 8;
-// Right-hand sides of one pass of the batched kernel over a tile, and the
-// width of the groups x is interleaved in for it (BatchPlan::xg).
+// Right-hand sides of one pass of the batched kernel over a tile at most, and
+// the widest group x is interleaved in for it (BatchPlan::xg); a plan's
+// group width is 4 or 8 (BatchPlan::gw).
 constexpr int kRhsGroup = 8;
+constexpr int kRhsGroupNarrow = 4;
 constexpr int kMaxBatchThreads = 64 * kMaxBatchTileItems;
 
 template <typename V, typename A>
@@ -63,7 +65,8 @@ struct BatchPlan {
     const A* x;               // column-major n x nrhs
     A* y;                     // column-major m x nrhs
     uint64_t ldx, ldy;
-    A* xg;                    // nrhs > 1: x interleaved in groups of kRhsGroup right-hand sides
+    A* xg;                    // nrhs > 1: x interleaved in groups of gw right-hand sides
+    uint32_t gw;              // group width: 4 or 8 (kRhsGroup)
     uint32_t xg_block0;       // the plan's first block of the interleave (batch_prologue)
     A* partial;               // nrhs x partial_stride
     uint32_t* counters;       // nrhs x counter_stride (zero between launches)
