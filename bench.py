@@ -521,6 +521,8 @@ def main():
     ap.add_argument("--block-size", type=int, default=16)
     ap.add_argument("--schedule", default="auto", choices=["auto", "stream", "blocked"])
     ap.add_argument("--item-nnz", type=int, default=0)
+    ap.add_argument("--tile-waves", type=int, default=0, help="stream schedule: waves per tile (0 = auto)")
+    ap.add_argument("--panels", type=int, default=0, help="stream schedule: x column panels 1, 2, 4 or 8 (0 = none)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE comparator")
@@ -531,6 +533,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (for scaling PMC byte counters)")
+    ap.add_argument("--calibrate-gather", action="store_true",
+                    help="only run 32M random 8-byte gathers from an 8 MiB FP64 vector (index_select), the "
+                         "request shape of the scattered x reads, for reading PMC counters per gather")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
@@ -547,6 +552,19 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 5}), flush=True)
+        return
+
+    if args.calibrate_gather:
+        n, k = 1 << 20, 1 << 25
+        g = torch.Generator(device="cpu").manual_seed(0)
+        src = torch.rand(n, dtype=torch.float64, generator=g).to("cuda:0")
+        idx = torch.randint(0, n, (k,), generator=g, dtype=torch.int64).to("cuda:0")
+        out = torch.empty(k, dtype=torch.float64, device="cuda:0")
+        for _ in range(5):
+            torch.index_select(src, 0, idx, out=out)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_gather_count": k, "x_bytes": n * 8, "index_bytes": k * 8, "out_bytes": k * 8,
+                          "launches": 5}), flush=True)
         return
 
     if args.mtx:
@@ -566,7 +584,8 @@ def main():
     m, n, nnz = d["m"], d["n"], int(d["row"].size)
     log(f"config {args.config}: {m}x{n}, nnz {nnz}, generated in {time.perf_counter() - t0:.1f}s")
     plan = cbspmv.Plan(m, n, d["row"], d["col"], d["val"], block_size=args.block_size, dtype=args.dtype,
-                       compensated=args.compensated, schedule=args.schedule, item_nnz=args.item_nnz)
+                       compensated=args.compensated, schedule=args.schedule, item_nnz=args.item_nnz,
+                       tile_waves=args.tile_waves, panels=args.panels)
     st = plan.stats()
     log(f"plan built in {st['build_seconds']:.2f}s: {st['gb_nnzb']} blocks (COO {st['coo_blocks']}, CSR "
         f"{st['csr_blocks']}, DENSE {st['dense_blocks']}), gather {st['gather_flag']}, schedule {plan.schedule()}")
