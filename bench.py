@@ -22,11 +22,13 @@ Other workloads (--workload), one JSON line each:
              FP32 native and compensated accumulation: GB/s of each, FP32/FP64
              throughput ratio, relative L2 error of each against the FP64
              oracle
-  autograd   the 524,288^2 power-law matrix (~20M nnz, FP32): forward time,
-             backward time per mode (dx only; dx + dvals over the stored
-             pattern in one fused pass, the line's value; dx + dvals in input
-             COO order, direct and fused + permutation), fwd+bwd / fwd,
-             gradient errors against FP64
+  autograd   the 524,288^2 power-law matrix (~20M nnz, FP32) through
+             torch.autograd (CBSparse with stored-order values,
+             cb_spmv_autograd + backward): the line's value is fwd+bwd / fwd
+             with both timed inside the training step; beside it the
+             inference forward, the whole step eager and as a captured graph,
+             the kernels alone, the value push after an optimiser step, the
+             input-COO-order backward variants, gradient errors against FP64
   batched    256 matrices (n log-uniform in [1e3, 2e5], 1-8 FP32 right-hand
              sides each): autotuned plans (16/32 blocks x 4/9 waves), one
              persistent launch, eager and hipGraph-replayed; B_alg summed
@@ -262,6 +264,48 @@ def bench_precision(args):
             "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded generator, cbspmv/gen.py config3)",
             "config": {"workload": "clustered_2M_50M_nnz_mixed_precision", "m": m, "n": n, "nnz": nnz},
             "results": res}
+
+
+def bench_decompose(args):
+    """Times sub-matrices of config 2 on their own, each as its own plan run
+    by the same kernel: the stencil half, the power-law half's entries near
+    the diagonal (|col - row| <= 4096, the columns the generator draws near
+    the row) and its uniformly scattered entries, and everything but the
+    scattered entries.  Their times against the whole matrix's show where
+    the launch spends its time and how far the parts overlap."""
+    import numpy as np
+    import torch
+
+    import cbspmv
+    from cbspmv import gen
+
+    dev = torch.device("cuda:0")
+    d = gen.config2()
+    m, n, r, c, v = d["m"], d["n"], d["row"], d["col"], d["val"]
+    half = m // 2
+    far = np.abs(c.astype(np.int64) - r.astype(np.int64)) > 4096
+    parts = {"whole": None, "stencil_half": r < half, "power_law_near": (r >= half) & ~far,
+             "power_law_scattered": (r >= half) & far, "all_but_scattered": ~far}
+    stream = torch.cuda.current_stream()
+    res = {}
+    for name, sel in parts.items():
+        rs, cs, vs = (r, c, v) if sel is None else (r[sel], c[sel], v[sel])
+        plan = cbspmv.Plan(m, n, rs, cs, vs, dtype=args.dtype)
+        vdt = plan.vector_dtype
+        x = torch.tensor(d["x"], dtype=vdt, device=dev)
+        y = torch.empty(m, dtype=vdt, device=dev)
+        plan.upload(stream)
+        ms = timed(lambda: plan.spmv(x, y, stream), args.steps, args.warmup, stream)
+        st = plan.stats()
+        res[name] = {"nnz": int(rs.size), "us": round(ms * 1e3, 2), "stream_slab_bytes": st["stream_bytes"],
+                     "slab_TB_s": round(st["stream_bytes"] / (ms * 1e-3) / 1e12, 2)}
+        log(f"{name}: {rs.size} entries, {ms * 1e3:.2f} us")
+        plan.close()
+        del x, y
+    res["overlap_us"] = round(res["all_but_scattered"]["us"] + res["power_law_scattered"]["us"]
+                              - res["whole"]["us"], 2)
+    return {"decomposition": "config2", "dtype": args.dtype, "steps": args.steps, "warmup": args.warmup,
+            "parts": res}
 
 
 def bench_autograd(args):
@@ -533,6 +577,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (for scaling PMC byte counters)")
+    ap.add_argument("--decompose", action="store_true",
+                    help="time sub-matrices of config 2 (stencil half, near and scattered power-law entries) "
+                         "each with the same kernel, at --dtype")
     ap.add_argument("--calibrate-gather", action="store_true",
                     help="only run 32M random 8-byte gathers from an 8 MiB FP64 vector (index_select), the "
                          "request shape of the scattered x reads, for reading PMC counters per gather")
@@ -567,6 +614,9 @@ def main():
                           "launches": 5}), flush=True)
         return
 
+    if args.decompose:
+        print(json.dumps(bench_decompose(args)), flush=True)
+        return
     if args.mtx:
         print(json.dumps(bench_mtx(args)), flush=True)
         return
