@@ -651,9 +651,9 @@ __device__ __forceinline__ void load_rhs(const double* p, double (&v)[R], uint32
 // Several right-hand sides at once: each chunk of keys and values is loaded
 // once and reduced for every right-hand side r < nr (at most R), so the slab
 // is read once per tile instead of once per right-hand side.  x comes
-// interleaved (a.x[col * R + r], written by batch_prologue) and is gathered
-// from global memory (L2) for every entry, the R values of a column with
-// back-to-back 16-byte loads (load_rhs); LDS holds the tile's line table to
+// interleaved (a.x[col * gw + r], written by batch_prologue) and is gathered
+// from global memory (L2) for every entry, four right-hand sides at a time
+// (load_rhs); LDS holds the tile's line table to
 // decode the column of an entry whose key names an LDS slot.  RHS r writes
 // y + r*ldy; pieces of cut rows go to partial + r*pstride and share one
 // ticket per split.  multi_item runs one item with one wave;
@@ -731,16 +731,24 @@ __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t i
         const int prev_off = wave_shift<kWaveShr1>(off[EPL - 1], -1);
         const int next_off = wave_shift<kWaveShl1>(off[0], -1);
         const bool carried = carry_off >= 0 && o0 == carry_off;
-        A xr[EPL][R];
+        // four right-hand sides at a time (a 16-byte load per entry for FP32):
+        // the x of eight for a chunk would hold 64 VGPRs, and with at most
+        // 128 per wave the kernel spilled; the second four read the same
+        // cache lines as the first
 #pragma unroll
-        for (int j = 0; j < EPL; ++j) load_rhs<R>(a.x + (size_t)col[j] * gw, xr[j], nr);
+        for (int h = 0; h < R; h += kRhsGroupNarrow) {
+          if ((uint32_t)h >= nr) break;
+          A xr[EPL][kRhsGroupNarrow];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
+          for (int j = 0; j < EPL; ++j) load_rhs<kRhsGroupNarrow>(a.x + (size_t)col[j] * gw + h, xr[j], nr - h);
+#pragma unroll
+          for (int q = 0; q < kRhsGroupNarrow; ++q) {
+            const int r = h + q;
             if ((uint32_t)r >= nr) break;
             Acc s[EPL];
 #pragma unroll
             for (int j = 0; j < EPL; ++j) {
-                const A xv = xr[j][r];
+                const A xv = xr[j][q];
                 if constexpr (COMP) s[j] = two_prod(widen(cur.v[j]), xv);
                 else s[j] = A(widen(cur.v[j])) * xv;
             }
@@ -766,6 +774,7 @@ __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t i
             }
             emit(valid && lane < L && next_off != off[EPL - 1], off[EPL - 1], T, false, r);
             carry[r] = read_lane(T, L);
+          }
         }
         carry_off = __builtin_amdgcn_readlane(off[EPL - 1], L);
         if (more) cur = nxt;

@@ -11,10 +11,10 @@ namespace cbs {
 namespace dev { struct bf16_t; }
 
 constexpr int kMaxStreamThreads = 64 * kMaxTileItems;   // one wave per tile item
-// The batched kernel has at most 8 waves per workgroup (each holds the x
-// values of eight right-hand sides for a chunk: ~210 VGPRs); the items of a
-// tile with more are dealt to its waves in turn.
-constexpr int kMaxBatchTileItems = 8;
+// The batched kernel has at most 16 waves per workgroup (at most 128 VGPRs
+// each, one workgroup per CU); the items of a tile with more are dealt to its
+// waves in turn.
+constexpr int kMaxBatchTileItems = 16;
 // Right-hand sides of one pass of the batched kernel over a tile at most, and
 // the widest group x is interleaved in for it (BatchPlan::xg); a plan's
 // group width is 4 or 8 (BatchPlan::gw).
