@@ -671,6 +671,19 @@ __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t i
     const uint32_t split_first = it.first_partial ? gptr(a.item_split_first)[item] : 0xffffffffu;
     const uint32_t split_last = it.last_partial ? gptr(a.item_split)[item] : 0xffffffffu;
     constexpr int kParts = COMP ? 2 : 1;
+    const uint32_t item_rows = min((uint32_t)a.m - it.row_base, omask + 1u);
+    // the first piece of a cut row: to the partial buffer of right-hand side r
+    auto put_partial_r = [&](uint32_t r, Acc v) {
+        const StreamSplit sp = load_split(a.splits, split_first);
+        const uint32_t slot = sp.partial_base + (item - sp.first_item);
+        gbl<A>* p = gptr(a.partial + (size_t)r * pstride);
+        if constexpr (COMP) {
+            p[2 * slot] = v.hi;
+            p[2 * slot + 1] = v.lo;
+        } else {
+            p[slot] = v;
+        }
+    };
 
     auto emit = [&](bool go, int off, Acc v, bool final_run, uint32_t r) {
         if (!go) return;
@@ -731,6 +744,24 @@ __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t i
         const int prev_off = wave_shift<kWaveShr1>(off[EPL - 1], -1);
         const int next_off = wave_shift<kWaveShl1>(off[0], -1);
         const bool carried = carry_off >= 0 && o0 == carry_off;
+        const bool carry_emit = carry_off >= 0 && !carried;   // the previous chunk's open run ended
+        // byte offsets of the rows whose runs end at each entry slot; the
+        // first piece of a cut row (offset 0 of an item with first_partial)
+        // goes to the partial buffer instead (row0_end)
+        uint32_t so[EPL];
+        bool row0_end[EPL];
+        bool any0 = false;
+#pragma unroll
+        for (int j = 0; j < EPL; ++j) {
+            const bool end = j < EPL - 1 ? valid && ends[j] : valid && lane < L && next_off != off[EPL - 1];
+            row0_end[j] = end && it.first_partial && off[j] == 0;
+            any0 = any0 || row0_end[j];
+            so[j] = end && !row0_end[j] ? (uint32_t)off[j] * (uint32_t)sizeof(A) : kOutOfRange;
+        }
+        const uint32_t so_carry = lane == 0 && carry_emit && !(it.first_partial && carry_off == 0)
+                                      ? (uint32_t)carry_off * (uint32_t)sizeof(A) : kOutOfRange;
+        // wave-uniform: the first piece of a cut row ends in this chunk
+        const bool row0 = it.first_partial && (__ballot(any0) != 0 || (carry_emit && carry_off == 0));
         // four right-hand sides at a time (a 16-byte load per entry for FP32):
         // the x of eight for a chunk would hold 64 VGPRs, and with at most
         // 128 per wave the kernel spilled; the second four read the same
@@ -752,27 +783,28 @@ __device__ __forceinline__ void multi_item(const StreamArgs<V, A>& a, uint32_t i
                 if constexpr (COMP) s[j] = two_prod(widen(cur.v[j]), xv);
                 else s[j] = A(widen(cur.v[j])) * xv;
             }
-            if (carry_off >= 0) {
-                if (carried) {
-                    if (lane == 0) s[0] += carry[r];
-                } else {
-                    emit(lane == 0, carry_off, carry[r], false, r);
-                }
-            }
+            if (carried && lane == 0) s[0] += carry[r];
 #pragma unroll
             for (int j = 1; j < EPL; ++j)
                 if (!ends[j - 1]) s[j] += s[j - 1];
             const Acc T = seg_scan_dpp(s[EPL - 1], off[EPL - 1]);
             const Acc prev_T = wave_shr1(T);
+            // runs that end in this chunk, and the run the previous chunk left
+            // open when it ends here: one buffer store per entry slot, lanes
+            // without a run end out of range
+            const uint32_t yb = (uint32_t)r * (uint32_t)ldy;
+            const rsrc_t ry = make_rsrc(a.y + yb + it.row_base, item_rows * (uint32_t)sizeof(A));
+            buffer_store<A>(finish(carry[r]), ry, so_carry);
             bool first = true;
 #pragma unroll
-            for (int j = 0; j < EPL - 1; ++j) {
-                Acc v = s[j];
-                if (first && prev_off == off[0]) v += prev_T;
-                emit(valid && ends[j], off[j], v, false, r);
-                first = first && !ends[j];
+            for (int j = 0; j < EPL; ++j) {
+                Acc v = j == EPL - 1 ? T : s[j];
+                if (j < EPL - 1 && first && prev_off == off[0]) v += prev_T;
+                buffer_store<A>(finish(v), ry, so[j]);
+                if (row0 && row0_end[j]) put_partial_r(r, v);   // the first piece of a cut row
+                if (j < EPL - 1) first = first && !ends[j];
             }
-            emit(valid && lane < L && next_off != off[EPL - 1], off[EPL - 1], T, false, r);
+            if (row0 && carry_emit && carry_off == 0 && lane == 0) put_partial_r(r, carry[r]);
             carry[r] = read_lane(T, L);
           }
         }
