@@ -1016,18 +1016,27 @@ __global__ __launch_bounds__(256) void batch_prologue(const BatchPlan<V, A>* pla
     const uint32_t gw = bp.gw;
     const uint32_t cols = (bp.nrhs + gw - 1) / gw * n;
     const uint32_t c0 = (blockIdx.x - bp.xg_block0) * kPrologueCols + threadIdx.x;
+    // all loads of the thread's columns first (a column past the end reads
+    // the last one and is not stored), then the stores
+    constexpr uint32_t K = kPrologueCols / 256;
+    A v[K][kMultiRhs];
 #pragma unroll
-    for (uint32_t k = 0; k < kPrologueCols / 256; ++k) {
-        const uint32_t c = c0 + k * 256;
-        if (c >= cols) break;
+    for (uint32_t k = 0; k < K; ++k) {
+        const uint32_t c = min(c0 + k * 256, cols - 1);
         const uint32_t g = c / n, col = c - g * n;
-        A v[kMultiRhs];
 #pragma unroll
         for (int j = 0; j < kMultiRhs; ++j) {
             const uint32_t r = g * gw + (uint32_t)j;
-            v[j] = (uint32_t)j < gw && r < bp.nrhs ? bp.x[(size_t)r * bp.ldx + col] : A(0);
+            v[k][j] = (uint32_t)j < gw && r < bp.nrhs ? gptr(bp.x)[(size_t)r * bp.ldx + col] : A(0);
         }
-        store_rhs<kMultiRhs>(bp.xg + (size_t)c * gw, v, min(bp.nrhs - g * gw, gw));
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        const uint32_t c = c0 + k * 256;
+        if (c < cols) {
+            const uint32_t g = c / n;
+            store_rhs<kMultiRhs>(bp.xg + (size_t)c * gw, v[k], min(bp.nrhs - g * gw, gw));
+        }
     }
 }
 
