@@ -84,28 +84,55 @@ __device__ __forceinline__ float dpp(float v) { return __uint_as_float(dpp32<CTR
 constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
 constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
 
-// One step of a segmented inclusive scan: add the value DPP brings in when it
-// comes from the same segment.  `seg` is non-decreasing along the wave, so an
-// equal key means every lane in between belongs to the same run.
-template <int CTRL, int ROWMASK, typename T>
-__device__ __forceinline__ void seg_step(T& v, int seg) {
-    const T pv = dpp<CTRL, ROWMASK>(v);
-    const int ps = (int)dpp32<CTRL, ROWMASK>((uint32_t)seg, 0xffffffffu);
-    if (ps == seg) v += pv;
+// DPP move for a value that a key comparison masks out where it has no
+// source: lanes without a source read 0 (bound_ctrl) and lanes in rows that
+// ROWMASK leaves out keep whatever the register held, so no old value has
+// to be written first.
+template <int CTRL, int ROWMASK = 0xf>
+__device__ __forceinline__ uint32_t dpp32_masked(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, ROWMASK, 0xf, true);
+}
+template <int CTRL, int ROWMASK = 0xf>
+__device__ __forceinline__ double dpp_masked(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = dpp32_masked<CTRL, ROWMASK>((uint32_t)b), hi = dpp32_masked<CTRL, ROWMASK>((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+template <int CTRL, int ROWMASK = 0xf>
+__device__ __forceinline__ float dpp_masked(float v) {
+    return __uint_as_float(dpp32_masked<CTRL, ROWMASK>(__float_as_uint(v)));
+}
+// Key of the source lane for a segmented scan step, keys stored as key + 1
+// (never 0): lanes without a source, and lanes in rows ROWMASK leaves out,
+// read 0, which matches no key.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_key(int key1) {
+    if constexpr (ROWMASK == 0xf) return (int)dpp32_masked<CTRL, 0xf>((uint32_t)key1);
+    else return (int)dpp32<CTRL, ROWMASK>((uint32_t)key1, 0u);
 }
 
-// Segmented inclusive scan over the 64 lanes for non-decreasing keys, all on
-// DPP: row_shr 1, 2, 4, 8 inside each 16-lane row, then row_bcast:15 into
-// rows 1 and 3 and row_bcast:31 into rows 2 and 3.  Run-end lanes end with
-// their run's sum.
+// One step of a segmented inclusive scan: add the value DPP brings in when it
+// comes from the same segment.  `seg1` (key + 1) is non-decreasing along the
+// wave, so an equal key means every lane in between belongs to the same run.
+template <int CTRL, int ROWMASK, typename T>
+__device__ __forceinline__ void seg_step(T& v, int seg1) {
+    const T pv = dpp_masked<CTRL, ROWMASK>(v);
+    if (dpp_key<CTRL, ROWMASK>(seg1) == seg1) v += pv;
+}
+
+// Segmented inclusive scan over the 64 lanes for non-decreasing keys (below
+// 0x7fffffff + 1), all on DPP: row_shr 1, 2, 4, 8 inside each 16-lane row,
+// then row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3.
+// Run-end lanes end with their run's sum.
 template <typename T>
 __device__ __forceinline__ T seg_scan_dpp(T v, int seg) {
-    seg_step<kRowShr1, 0xf>(v, seg);
-    seg_step<kRowShr2, 0xf>(v, seg);
-    seg_step<kRowShr4, 0xf>(v, seg);
-    seg_step<kRowShr8, 0xf>(v, seg);
-    seg_step<kRowBcast15, 0xa>(v, seg);
-    seg_step<kRowBcast31, 0xc>(v, seg);
+    const int seg1 = (int)((uint32_t)seg + 1u);
+    seg_step<kRowShr1, 0xf>(v, seg1);
+    seg_step<kRowShr2, 0xf>(v, seg1);
+    seg_step<kRowShr4, 0xf>(v, seg1);
+    seg_step<kRowShr8, 0xf>(v, seg1);
+    seg_step<kRowBcast15, 0xa>(v, seg1);
+    seg_step<kRowBcast31, 0xc>(v, seg1);
     return v;
 }
 

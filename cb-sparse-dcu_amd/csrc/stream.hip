@@ -61,18 +61,18 @@ template <int CTRL, int ROWMASK>
 __device__ __forceinline__ f2 dpp_f2(f2 v) { return f2{dpp<CTRL, ROWMASK>(v.hi), dpp<CTRL, ROWMASK>(v.lo)}; }
 
 template <int CTRL, int ROWMASK>
-__device__ __forceinline__ void seg_step2(f2& v, int seg) {
-    const f2 pv = dpp_f2<CTRL, ROWMASK>(v);
-    const int ps = (int)dpp32<CTRL, ROWMASK>((uint32_t)seg, 0xffffffffu);
-    if (ps == seg) v += pv;
+__device__ __forceinline__ void seg_step2(f2& v, int seg1) {
+    const f2 pv = f2{dpp_masked<CTRL, ROWMASK>(v.hi), dpp_masked<CTRL, ROWMASK>(v.lo)};
+    if (dpp_key<CTRL, ROWMASK>(seg1) == seg1) v += pv;
 }
 __device__ __forceinline__ f2 seg_scan_dpp(f2 v, int seg) {
-    seg_step2<kRowShr1, 0xf>(v, seg);
-    seg_step2<kRowShr2, 0xf>(v, seg);
-    seg_step2<kRowShr4, 0xf>(v, seg);
-    seg_step2<kRowShr8, 0xf>(v, seg);
-    seg_step2<kRowBcast15, 0xa>(v, seg);
-    seg_step2<kRowBcast31, 0xc>(v, seg);
+    const int seg1 = (int)((uint32_t)seg + 1u);
+    seg_step2<kRowShr1, 0xf>(v, seg1);
+    seg_step2<kRowShr2, 0xf>(v, seg1);
+    seg_step2<kRowShr4, 0xf>(v, seg1);
+    seg_step2<kRowShr8, 0xf>(v, seg1);
+    seg_step2<kRowBcast15, 0xa>(v, seg1);
+    seg_step2<kRowBcast31, 0xc>(v, seg1);
     return v;
 }
 
