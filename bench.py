@@ -15,6 +15,9 @@ cpu_baseline = the reference's own CPU path (check_results_consistency, its
              sequential FP64 SpMV, compiled from the reference sources into
              oracle/_ref), timed on the host on the same matrix; the oracle's
              restatement of the same loop when oracle/_ref is absent
+extensions = the precision, autograd and batched workloads below, run after
+             the default line's own measurement in the same process (their
+             metric, value and results each); --no-extensions skips them
 
 Other workloads (--workload), one JSON line each:
   precision  the 2,097,152^2 block-clustered matrix (~50M nnz, values scaled
@@ -207,7 +210,10 @@ def timed(fn, steps, warmup, stream):
     return a.elapsed_time(b) / steps
 
 
-def bench_precision(args):
+def bench_precision(args, d2=None):
+    """FP64 / FP32 / BF16 storage on the block-clustered matrix, then FP64
+    and FP32 on the 30M-entry matrix of the default line (d2: that matrix
+    when the caller has generated it already)."""
     import numpy as np
     import torch
 
@@ -242,7 +248,8 @@ def bench_precision(args):
     speed = res["f64"]["us"] / res["f32"]["us"]
     # the same comparison on the 30M-entry matrix of the default line, whose
     # FP32 x (4 MB) fits one XCD's L2 where the FP64 x (8 MB) does not
-    d2 = gen.config2()
+    if d2 is None:
+        d2 = gen.config2()
     m2, n2, nnz2 = d2["m"], d2["n"], int(d2["row"].size)
     cfg2 = {}
     for dtype in ("f64", "f32"):
@@ -272,7 +279,15 @@ def bench_decompose(args):
     the diagonal (|col - row| <= 4096, the columns the generator draws near
     the row) and its uniformly scattered entries, and everything but the
     scattered entries.  Their times against the whole matrix's show where
-    the launch spends its time and how far the parts overlap."""
+    the launch spends its time and how far the parts overlap.
+
+    Then the whole matrix again with only the scattered entries' columns
+    moved (rows, values and every other entry unchanged; each a valid
+    matrix whose product the kernel computes in full): folded into the
+    first 131,072 or 524,288 columns (1 or 4 MB of FP64 x, gathered from
+    L2), or next to the diagonal (row + 4097 + col mod 2048, reflected at
+    the edge: columns the tile stages in LDS).  The three times bound what
+    the scattered gathers cost, and what any placement of them could gain."""
     import numpy as np
     import torch
 
@@ -286,10 +301,18 @@ def bench_decompose(args):
     far = np.abs(c.astype(np.int64) - r.astype(np.int64)) > 4096
     parts = {"whole": None, "stencil_half": r < half, "power_law_near": (r >= half) & ~far,
              "power_law_scattered": (r >= half) & far, "all_but_scattered": ~far}
+    r64, c64 = r.astype(np.int64), c.astype(np.int64)
+    near = r64 + 4097 + c64 % 2048
+    near = np.where(near < n, near, r64 - 4097 - c64 % 2048)
+    moved = {"scattered_cols_in_1MB": np.where(far, c64 % 131072, c64),
+             "scattered_cols_in_4MB": np.where(far, c64 % 524288, c64),
+             "scattered_cols_near_diagonal": np.where(far, near, c64)}
     stream = torch.cuda.current_stream()
     res = {}
-    for name, sel in parts.items():
+    for name, sel in list(parts.items()) + [(k, None) for k in moved]:
         rs, cs, vs = (r, c, v) if sel is None else (r[sel], c[sel], v[sel])
+        if name in moved:
+            cs = moved[name].astype(np.int32)
         plan = cbspmv.Plan(m, n, rs, cs, vs, dtype=args.dtype)
         vdt = plan.vector_dtype
         x = torch.tensor(d["x"], dtype=vdt, device=dev)
@@ -545,6 +568,36 @@ def bench_mtx(args):
             "gflops": round(met.gflops, 2), "rel_l2_vs_fp64_oracle": err}
 
 
+def run_extensions(args, d2):
+    """The three extension workloads, measured in the same run as the default
+    line (so their figures come from the same process and GPU): the
+    precision ratio on the block-clustered matrix (and on the default line's
+    matrix, d2), fwd+bwd / fwd through torch.autograd on the 512K^2 matrix,
+    and the 256-matrix batched graph replay.  Each entry keeps its workload's
+    metric, value and results; a workload that raises is reported with its
+    error text instead (the default line's own figures do not depend on
+    them)."""
+    import traceback
+
+    import torch
+
+    keep = ("metric", "value", "unit", "ms_per_step", "dtype", "config", "results")
+    ext = {}
+    for name, fn in (("precision", lambda: bench_precision(args, d2)), ("autograd", lambda: bench_autograd(args)),
+                     ("batched", lambda: bench_batched(args))):
+        t0 = time.perf_counter()
+        try:
+            r = fn()
+            ext[name] = {k: r[k] for k in keep if k in r}
+        except Exception as e:   # reported in the line, never hidden
+            traceback.print_exc()
+            ext[name] = {"error": f"{type(e).__name__}: {e}"}
+        ext[name]["seconds"] = round(time.perf_counter() - t0, 1)
+        log(f"extension {name}: {ext[name].get('value')} in {ext[name]['seconds']} s")
+        torch.cuda.empty_cache()
+    return ext
+
+
 def read_traffic(workload_key):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -570,6 +623,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE comparator")
+    ap.add_argument("--no-extensions", action="store_true",
+                    help="default line only: skip the precision, autograd and batched workloads it also reports")
     ap.add_argument("--workload", default="spmv", choices=["spmv", "precision", "autograd", "batched"])
     ap.add_argument("--mtx", default=None,
                     help="run the reference driver's protocol on a Matrix Market file (main.hip.cpp:46-75)")
@@ -718,8 +773,11 @@ def main():
         "cpu_baseline": cpu,
         "vendor_baseline": None if args.no_vendor else vendor_baseline(d, args.dtype, 200, ms),
     }
-    print(json.dumps(out), flush=True)
     plan.close()
+    del x, y
+    if not args.no_extensions and args.config == 2 and args.dtype == "f64":
+        out["extensions"] = run_extensions(args, d)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
