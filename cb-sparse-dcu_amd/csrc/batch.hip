@@ -428,6 +428,19 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
     (void)hipEventCreate(&e1);
     int rc = CBSPMV_OK;
     double best_t = 1e300;
+    // The stream schedule reads the strip entries, not the blocks: without
+    // DENSE blocks (whose zero fill depends on the block size) two block
+    // sizes give the same entries, item cuts and staged lines, so a
+    // candidate whose stream plan has the shape of one already timed is
+    // skipped (the first one stays).
+    struct Shape {
+        int tile_items;
+        size_t items, slab, lines;
+        bool operator==(const Shape& b) const {
+            return tile_items == b.tile_items && items == b.items && slab == b.slab && lines == b.lines;
+        }
+    };
+    std::vector<Shape> timed;
     for (int c = 0; c < ncandidates && rc == CBSPMV_OK; ++c) {
         cbspmv_opts oc = o;
         oc.block_size = candidates[2 * c];
@@ -438,6 +451,19 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
         cbspmv_plan* p = nullptr;
         rc = cbspmv_build(m, n, nnz, row, col, val, &oc, &p);
         if (rc) break;
+        if (p->schedule == CBSPMV_SCHED_STREAM) {
+            const cbs::Plan& h = p->host;
+            const bool dense = std::any_of(h.type.begin(), h.type.end(),
+                                           [](uint8_t t) { return (t & cbs::kTypeMask) == cbs::kDENSE; });
+            if (!dense) {
+                const Shape sh{h.o.tile_items, h.items.size(), h.s_slab.size(), h.tile_lines.size()};
+                if (std::find(timed.begin(), timed.end(), sh) != timed.end()) {
+                    cbspmv_destroy(p);
+                    continue;
+                }
+                timed.push_back(sh);
+            }
+        }
         if ((rc = cbs::upload_plan(*p, s))) { cbspmv_destroy(p); break; }
         // Timed the way a batch runs it: K copies of the plan in one batch
         // (K sized so that the copies' work units fill the GPU several

@@ -268,7 +268,11 @@ void cbspmv_batch_destroy(cbspmv_batch* batch);
  * lists (block_size, waves) pairs; waves sets both the blocked schedule's
  * waves per workgroup and the stream schedule's tile waves, 0 keeps both
  * as `base` has them (its auto tile waves by default).
- * NULL = {16,0},{32,0},{16,4},{16,9},{32,4},{32,9}.
+ * NULL = {16,0},{32,0},{16,4},{16,9},{32,4},{32,9}.  A stream-schedule
+ * candidate without DENSE blocks whose stream plan has the shape of one
+ * already timed (same tile waves, items, slab bytes and staged lines: the
+ * block size then changes nothing the stream kernel reads) is not timed;
+ * the earlier candidate stands for it.
  * Allocates device memory for its test vectors. */
 int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const double* val,
                     const cbspmv_opts* base, const int* candidates, int ncandidates,

@@ -213,3 +213,11 @@ def test_autotune_returns_a_correct_plan(gpu):
     _check([(m, n, r, c, v)], [X], [Y], 1e-6)
     b.close()
     p.close()
+    # without DENSE blocks both block sizes give one stream plan: the second
+    # candidate is not timed, whichever would have measured faster
+    for first in (16, 32):
+        other = 48 - first
+        p, _ = autotune(m, n, r, c, v, nrhs=1, candidates=[(first, 9), (other, 9)], dtype="f32")
+        assert p.stats()["dense_blocks"] == 0
+        assert p.stats()["gb_m"] == (m + first - 1) // first
+        p.close()
