@@ -15,9 +15,10 @@ cpu_baseline = the reference's own CPU path (check_results_consistency, its
              sequential FP64 SpMV, compiled from the reference sources into
              oracle/_ref), timed on the host on the same matrix; the oracle's
              restatement of the same loop when oracle/_ref is absent
-extensions = the precision, autograd and batched workloads below, run after
-             the default line's own measurement in the same process (their
-             metric, value and results each); --no-extensions skips them
+extensions = the precision, autograd and batched workloads below, each run
+             as a child process after the default line's own measurement
+             (their metric, value and results each); --no-extensions skips
+             them
 
 Other workloads (--workload), one JSON line each:
   precision  the 2,097,152^2 block-clustered matrix (~50M nnz, values scaled
@@ -210,10 +211,9 @@ def timed(fn, steps, warmup, stream):
     return a.elapsed_time(b) / steps
 
 
-def bench_precision(args, d2=None):
+def bench_precision(args):
     """FP64 / FP32 / BF16 storage on the block-clustered matrix, then FP64
-    and FP32 on the 30M-entry matrix of the default line (d2: that matrix
-    when the caller has generated it already)."""
+    and FP32 on the 30M-entry matrix of the default line."""
     import numpy as np
     import torch
 
@@ -248,8 +248,7 @@ def bench_precision(args, d2=None):
     speed = res["f64"]["us"] / res["f32"]["us"]
     # the same comparison on the 30M-entry matrix of the default line, whose
     # FP32 x (4 MB) fits one XCD's L2 where the FP64 x (8 MB) does not
-    if d2 is None:
-        d2 = gen.config2()
+    d2 = gen.config2()
     m2, n2, nnz2 = d2["m"], d2["n"], int(d2["row"].size)
     cfg2 = {}
     for dtype in ("f64", "f32"):
@@ -568,33 +567,35 @@ def bench_mtx(args):
             "gflops": round(met.gflops, 2), "rel_l2_vs_fp64_oracle": err}
 
 
-def run_extensions(args, d2):
-    """The three extension workloads, measured in the same run as the default
-    line (so their figures come from the same process and GPU): the
-    precision ratio on the block-clustered matrix (and on the default line's
-    matrix, d2), fwd+bwd / fwd through torch.autograd on the 512K^2 matrix,
-    and the 256-matrix batched graph replay.  Each entry keeps its workload's
-    metric, value and results; a workload that raises is reported with its
-    error text instead (the default line's own figures do not depend on
-    them)."""
-    import traceback
-
-    import torch
+def run_extensions(args):
+    """The three extension workloads, measured after the default line in the
+    same run: the precision ratio on the block-clustered matrix (and on the
+    default line's matrix), fwd+bwd / fwd through torch.autograd on the
+    512K^2 matrix, and the 256-matrix batched graph replay.  Each runs as a
+    child process (`bench.py --workload W` with this run's steps and
+    warm-up, at least 200 and 20: a workload's launches take at most a few
+    milliseconds each), one after the other, so that a workload that fails, even by
+    a crash, cannot take the default line with it; each entry keeps the
+    workload's metric, value and results, or its exit status and the end of
+    its error output."""
+    import subprocess
 
     keep = ("metric", "value", "unit", "ms_per_step", "dtype", "config", "results")
     ext = {}
-    for name, fn in (("precision", lambda: bench_precision(args, d2)), ("autograd", lambda: bench_autograd(args)),
-                     ("batched", lambda: bench_batched(args))):
+    for name in ("precision", "autograd", "batched"):
         t0 = time.perf_counter()
-        try:
-            r = fn()
+        cmd = [sys.executable, os.path.abspath(__file__), "--workload", name, "--steps", str(max(args.steps, 200)),
+               "--warmup", str(max(args.warmup, 20))]
+        proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+        sys.stderr.write(proc.stderr)
+        lines = [ln for ln in proc.stdout.splitlines() if ln.startswith("{")]
+        if proc.returncode == 0 and lines:
+            r = json.loads(lines[-1])
             ext[name] = {k: r[k] for k in keep if k in r}
-        except Exception as e:   # reported in the line, never hidden
-            traceback.print_exc()
-            ext[name] = {"error": f"{type(e).__name__}: {e}"}
+        else:   # reported in the line, never hidden
+            ext[name] = {"error": f"exit status {proc.returncode}", "stderr_tail": proc.stderr[-2000:]}
         ext[name]["seconds"] = round(time.perf_counter() - t0, 1)
         log(f"extension {name}: {ext[name].get('value')} in {ext[name]['seconds']} s")
-        torch.cuda.empty_cache()
     return ext
 
 
@@ -776,7 +777,8 @@ def main():
     plan.close()
     del x, y
     if not args.no_extensions and args.config == 2 and args.dtype == "f64":
-        out["extensions"] = run_extensions(args, d)
+        torch.cuda.empty_cache()
+        out["extensions"] = run_extensions(args)
     print(json.dumps(out), flush=True)
 
 
