@@ -28,6 +28,16 @@ constexpr int kBatchMaxRhs = 16;
 constexpr uint32_t kAutotuneUnits = 2048;
 constexpr uint32_t kAutotuneMaxCopies = 64;
 constexpr size_t kAutotuneYBytes = size_t(1) << 30;   // bound on the copies' y
+// autotuner: batch shape when the base options leave it open
+constexpr int kBatchItemEntries = 2048;   // entries per item at least
+constexpr int kBatchMultiTileKib = 4;     // LDS x cache of plans with several right-hand sides
+// autotuner: a later candidate replaces the one kept only when it is this
+// much faster.  A candidate is timed against copies of itself, which
+// predicts its cost inside a mixed batch only roughly: on the 256 matrices
+// of config 4, keeping every candidate that timed faster at all picked 9
+// waves for 67 matrices and made the batch slower (1.63 ms) than 16 waves
+// for all of them (1.56 ms).
+constexpr double kAutotuneMargin = 0.05;
 }  // namespace cbs
 
 struct cbspmv_batch {
@@ -428,48 +438,12 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
     (void)hipEventCreate(&e1);
     int rc = CBSPMV_OK;
     double best_t = 1e300;
-    // The stream schedule reads the strip entries, not the blocks: without
-    // DENSE blocks (whose zero fill depends on the block size) two block
-    // sizes give the same entries, item cuts and staged lines, so a
-    // candidate whose stream plan has the shape of one already timed is
-    // skipped (the first one stays).
-    struct Shape {
-        int tile_items;
-        size_t items, slab, lines;
-        bool operator==(const Shape& b) const {
-            return tile_items == b.tile_items && items == b.items && slab == b.slab && lines == b.lines;
-        }
-    };
-    std::vector<Shape> timed;
-    for (int c = 0; c < ncandidates && rc == CBSPMV_OK; ++c) {
-        cbspmv_opts oc = o;
-        oc.block_size = candidates[2 * c];
-        if (candidates[2 * c + 1] > 0) {
-            oc.waves_per_block = candidates[2 * c + 1];
-            oc.stream_tile_waves = candidates[2 * c + 1];
-        }
-        cbspmv_plan* p = nullptr;
-        rc = cbspmv_build(m, n, nnz, row, col, val, &oc, &p);
-        if (rc) break;
-        if (p->schedule == CBSPMV_SCHED_STREAM) {
-            const cbs::Plan& h = p->host;
-            const bool dense = std::any_of(h.type.begin(), h.type.end(),
-                                           [](uint8_t t) { return (t & cbs::kTypeMask) == cbs::kDENSE; });
-            if (!dense) {
-                const Shape sh{h.o.tile_items, h.items.size(), h.s_slab.size(), h.tile_lines.size()};
-                if (std::find(timed.begin(), timed.end(), sh) != timed.end()) {
-                    cbspmv_destroy(p);
-                    continue;
-                }
-                timed.push_back(sh);
-            }
-        }
-        if ((rc = cbs::upload_plan(*p, s))) { cbspmv_destroy(p); break; }
-        // Timed the way a batch runs it: K copies of the plan in one batch
-        // (K sized so that the copies' work units fill the GPU several
-        // times over), costed at 1/K of that launch, i.e. its throughput
-        // beside other matrices rather than its latency alone on the GPU.
-        // Plans a batch does not take: one launch per right-hand side.
+    // Times an uploaded plan the way a batch runs it: K copies of the plan
+    // in one batch (K sized so that the copies' work units fill the GPU
+    // several times over), costed at 1/K of that launch, i.e. its throughput
+    // beside other matrices rather than its latency alone on the GPU.  Plans
+    // a batch does not take: one launch per right-hand side.
+    auto time_plan = [&](cbspmv_plan* p) -> double {
         const uint32_t units = std::max<uint32_t>(1u, (uint32_t)(p->host.tiles.size() / 2));
         const size_t y_bytes = std::max<size_t>(1, (size_t)m * nrhs * eb);
         const int K = (int)std::max<size_t>(
@@ -503,16 +477,66 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
         (void)hipEventSynchronize(e1);
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        const double t = ms / kReps / per_run;
         if (one) cbspmv_batch_destroy(one);
         if (dyk) (void)hipFree(dyk);
-        if (rc == CBSPMV_OK && t < best_t) {
+        return ms / kReps / per_run;
+    };
+    // Builds, times and keeps or destroys one candidate.
+    auto try_candidate = [&](const cbspmv_opts& oc, bool dedupe, std::vector<cbs::StreamPlanShape>& timed) {
+        cbspmv_plan* p = nullptr;
+        rc = cbspmv_build(m, n, nnz, row, col, val, &oc, &p);
+        if (rc) return;
+        if (dedupe && p->schedule == CBSPMV_SCHED_STREAM) {
+            const cbs::StreamPlanShape sh = cbs::stream_plan_shape(p->host);
+            if (!sh.dense) {
+                if (std::find(timed.begin(), timed.end(), sh) != timed.end()) {
+                    cbspmv_destroy(p);
+                    return;
+                }
+                timed.push_back(sh);
+            }
+        }
+        if ((rc = cbs::upload_plan(*p, s))) {
+            cbspmv_destroy(p);
+            return;
+        }
+        const double t = time_plan(p);
+        if (rc == CBSPMV_OK && t < (*best ? best_t * (1.0 - cbs::kAutotuneMargin) : best_t)) {
             if (*best) cbspmv_destroy(*best);
             *best = p;
             best_t = t;
         } else {
             cbspmv_destroy(p);
         }
+    };
+    // The stream schedule reads the strip entries, not the blocks: without
+    // DENSE blocks (whose zero fill depends on the block size) two block
+    // sizes give the same entries, item cuts and staged lines, so a
+    // candidate whose stream plan has the shape of one already timed is
+    // skipped (the first one stays).
+    std::vector<cbs::StreamPlanShape> timed;
+    const int cus = cbs::device_shape().cus;
+    for (int c = 0; c < ncandidates && rc == CBSPMV_OK; ++c) {
+        cbspmv_opts oc = o;
+        oc.block_size = candidates[2 * c];
+        if (candidates[2 * c + 1] > 0) {
+            oc.waves_per_block = candidates[2 * c + 1];
+            oc.stream_tile_waves = candidates[2 * c + 1];
+        }
+        // Batch shape, where `base` leaves it open.  Items: the automatic
+        // size fills the GPU with one matrix's tiles (down to 1024 entries
+        // per item); inside a batch, items of at least kBatchItemEntries
+        // start up less often (256 matrices of config 4: 1.68 -> 1.57 ms;
+        // 3072: 1.58 ms).  LDS: with several right-hand sides the tile's x
+        // is gathered, not staged, so its LDS only decodes staged keys and
+        // the smallest cache serves (1.567 -> 1.559 ms).
+        if (o.stream_item_nnz == 0) {
+            const int waves = oc.stream_tile_waves > 0 ? oc.stream_tile_waves : cbs::kWideTileItemsOpt;
+            const long long fill = ((long long)nnz + (long long)cus * waves - 1) / ((long long)cus * waves);
+            oc.stream_item_nnz = (int)std::max<long long>(cbs::kBatchItemEntries, fill);
+        }
+        if (nrhs > 1 && o.stream_tile_kib == 0) oc.stream_tile_kib = cbs::kBatchMultiTileKib;
+        try_candidate(oc, true, timed);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -108,6 +108,10 @@ constexpr int64_t kPanelBytes = 2ll << 20;      // panel width target (bytes of 
 // stream_panels = 0 picks panels by the size of x only when this is set
 constexpr bool kPanelsByDefault = false;
 
+// Automatic item size: the smallest that cuts the stream into one round of
+// tiles, but at least this many entries per item.
+constexpr int kMinItemEntries = 1024;
+
 struct StreamSplit {       // one per row cut into pieces
     uint32_t row;          // matrix row
     uint32_t first_item;   // item holding piece 0
@@ -181,6 +185,28 @@ struct Plan {
     std::vector<uint32_t> panel_item_ptr;  // panel q's items: [ptr[q], ptr[q + 1])
     std::vector<uint8_t> panel_mask;       // per row: panels holding entries of the row
 };
+
+// What the stream kernel reads of a plan, in brief: two plans of one matrix
+// with the same shape and no DENSE blocks (whose zero fill depends on the
+// block size) stream the same entries in the same items and tiles.
+struct StreamPlanShape {
+    int tile_items = 0;
+    size_t items = 0, slab = 0, lines = 0;
+    bool dense = false;
+    bool operator==(const StreamPlanShape& b) const {
+        return tile_items == b.tile_items && items == b.items && slab == b.slab && lines == b.lines &&
+               dense == b.dense;
+    }
+};
+inline StreamPlanShape stream_plan_shape(const Plan& h) {
+    StreamPlanShape sh;
+    sh.tile_items = h.o.tile_items;
+    sh.items = h.items.size();
+    sh.slab = h.s_slab.size();
+    sh.lines = h.tile_lines.size();
+    for (uint8_t t : h.type) sh.dense = sh.dense || (t & kTypeMask) == kDENSE;
+    return sh;
+}
 
 // Builder (builder.cpp): bit-exact with coo2block_gather / coo2block_comp.
 int build_plan(Plan& p, int m, int n, int nnz, const int* row, const int* col,

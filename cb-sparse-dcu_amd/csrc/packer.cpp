@@ -315,7 +315,7 @@ void layout_tiles(Plan& p, const StreamShape& sh, const Entries& E, std::vector<
         // that few CUs are left without a tile (30M entries in FP64: 256
         // tiles instead of 249)
         const int64_t max_items = (int64_t)p.o.tiles * p.o.tile_items;
-        int64_t lo = std::max<int64_t>(1024, total / max_items + 1), hi = lo;
+        int64_t lo = std::max<int64_t>(kMinItemEntries, total / max_items + 1), hi = lo;
         cut(hi);
         bool fits = (int64_t)pieces.size() <= max_items;
         for (int round = 0; round < 64 && !fits && hi < (1 << 22); ++round) {

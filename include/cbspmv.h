@@ -264,7 +264,8 @@ void cbspmv_batch_destroy(cbspmv_batch* batch);
  * each (enough copies to fill the GPU; 3 warm-up, mean of 20; the time is
  * per copy, i.e. what the matrix costs inside a full batch; a plan a batch
  * does not take is timed with nrhs single launches) and returns the
- * fastest, destroying the others.  *best_ms is that per-copy time.  `candidates`
+ * fastest, destroying the others; a later candidate replaces an earlier
+ * one only when it is at least 5% faster, so list the preferred one first.  *best_ms is that per-copy time.  `candidates`
  * lists (block_size, waves) pairs; waves sets both the blocked schedule's
  * waves per workgroup and the stream schedule's tile waves, 0 keeps both
  * as `base` has them (its auto tile waves by default).
@@ -272,7 +273,11 @@ void cbspmv_batch_destroy(cbspmv_batch* batch);
  * candidate without DENSE blocks whose stream plan has the shape of one
  * already timed (same tile waves, items, slab bytes and staged lines: the
  * block size then changes nothing the stream kernel reads) is not timed;
- * the earlier candidate stands for it.
+ * the earlier candidate stands for it.  Options `base` leaves at 0 get a
+ * batch shape: items of at least 2048 entries (stream_item_nnz; the
+ * automatic size fills the GPU with one matrix and goes down to 1024), and
+ * with nrhs > 1 a 4 KiB LDS x cache (stream_tile_kib; several right-hand
+ * sides gather x instead of staging it).
  * Allocates device memory for its test vectors. */
 int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const double* val,
                     const cbspmv_opts* base, const int* candidates, int ncandidates,

@@ -214,10 +214,27 @@ def test_autotune_returns_a_correct_plan(gpu):
     b.close()
     p.close()
     # without DENSE blocks both block sizes give one stream plan: the second
-    # candidate is not timed, whichever would have measured faster
+    # candidate is not timed, whichever would have measured faster (a fixed
+    # item size leaves out the item-size stage)
     for first in (16, 32):
         other = 48 - first
-        p, _ = autotune(m, n, r, c, v, nrhs=1, candidates=[(first, 9), (other, 9)], dtype="f32")
+        p, _ = autotune(m, n, r, c, v, nrhs=1, candidates=[(first, 9), (other, 9)], dtype="f32", item_nnz=1024)
         assert p.stats()["dense_blocks"] == 0
         assert p.stats()["gb_m"] == (m + first - 1) // first
         p.close()
+    # options left open get the batch shape: items of about 2048 entries
+    # (the automatic size here would be about 1024) and, with several
+    # right-hand sides, a 4 KiB LDS cache; the plan is correct in a batch
+    p, _ = autotune(m, n, r, c, v, nrhs=3, candidates=[(16, 16)], dtype="f32")
+    items = p.export("stream_items").reshape(-1, 4)
+    mean = float((items[:, 1] - items[:, 0]).mean())
+    assert 1800 <= mean <= 2600, mean
+    assert int(p.export("stream_tile_shape")[1]) == 4096
+    X = torch.tensor(np.random.default_rng(8).uniform(-1, 1, (3, n)), dtype=torch.float32, device=gpu)
+    Y = torch.zeros((3, m), dtype=torch.float32, device=gpu)
+    b = Batch([p])
+    b.spmv([X], [Y])
+    torch.cuda.synchronize()
+    _check([(m, n, r, c, v)], [X], [Y], 1e-6)
+    b.close()
+    p.close()
