@@ -29,14 +29,16 @@ constexpr uint32_t kAutotuneUnits = 2048;
 constexpr uint32_t kAutotuneMaxCopies = 64;
 constexpr size_t kAutotuneYBytes = size_t(1) << 30;   // bound on the copies' y
 // autotuner: batch shape when the base options leave it open
-constexpr int kBatchItemEntries = 2048;   // entries per item at least
-constexpr int kBatchMultiTileKib = 4;     // LDS x cache of plans with several right-hand sides
+constexpr int kBatchTileWaves = 8;        // stream tile waves
+constexpr int kBatchItemEntries = 2048;   // entries per item
+constexpr int kBatchTileKib = 32;         // LDS x cache, one right-hand side
+constexpr int kBatchMultiTileKib = 4;     // LDS x cache, several right-hand sides
 // autotuner: a later candidate replaces the one kept only when it is this
 // much faster.  A candidate is timed against copies of itself, which
 // predicts its cost inside a mixed batch only roughly: on the 256 matrices
-// of config 4, keeping every candidate that timed faster at all picked 9
-// waves for 67 matrices and made the batch slower (1.63 ms) than 16 waves
-// for all of them (1.56 ms).
+// of config 4, when candidates still chose the stream tile waves, keeping
+// every one that timed faster at all picked 9 waves for 67 matrices and
+// made the batch slower (1.63 ms) than 16 waves for all of them (1.56 ms).
 constexpr double kAutotuneMargin = 0.05;
 }  // namespace cbs
 
@@ -515,27 +517,34 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
     // candidate whose stream plan has the shape of one already timed is
     // skipped (the first one stays).
     std::vector<cbs::StreamPlanShape> timed;
-    const int cus = cbs::device_shape().cus;
     for (int c = 0; c < ncandidates && rc == CBSPMV_OK; ++c) {
         cbspmv_opts oc = o;
         oc.block_size = candidates[2 * c];
-        if (candidates[2 * c + 1] > 0) {
-            oc.waves_per_block = candidates[2 * c + 1];
-            oc.stream_tile_waves = candidates[2 * c + 1];
-        }
-        // Batch shape, where `base` leaves it open.  Items: the automatic
-        // size fills the GPU with one matrix's tiles (down to 1024 entries
-        // per item); inside a batch, items of at least kBatchItemEntries
-        // start up less often (256 matrices of config 4: 1.68 -> 1.57 ms;
-        // 3072: 1.58 ms).  LDS: with several right-hand sides the tile's x
-        // is gathered, not staged, so its LDS only decodes staged keys and
-        // the smallest cache serves (1.567 -> 1.559 ms).
-        if (o.stream_item_nnz == 0) {
-            const int waves = oc.stream_tile_waves > 0 ? oc.stream_tile_waves : cbs::kWideTileItemsOpt;
-            const long long fill = ((long long)nnz + (long long)cus * waves - 1) / ((long long)cus * waves);
-            oc.stream_item_nnz = (int)std::max<long long>(cbs::kBatchItemEntries, fill);
-        }
-        if (nrhs > 1 && o.stream_tile_kib == 0) oc.stream_tile_kib = cbs::kBatchMultiTileKib;
+        if (candidates[2 * c + 1] > 0) oc.waves_per_block = candidates[2 * c + 1];
+        // Batch shape, where `base` leaves it open.  The batched launch runs
+        // every unit with the widest tile of the batch and as much LDS as the
+        // largest cache, so the stream tile shape is one choice for all of
+        // its plans, not a per-matrix one (a 9- or 16-wave plan among 8-wave
+        // ones leaves waves idle or halves the workgroups per CU):
+        //  * 8 tile waves with a cache of at most 32 KiB, so that two
+        //    workgroups share a CU and a unit's slowest wave holds up only
+        //    half of it (256 matrices of config 4: 1.557 -> 1.466 ms against
+        //    16 waves; 4 waves 1.475, 6 1.541, 10 1.685 ms);
+        //  * with several right-hand sides x is gathered, not staged, so the
+        //    cache only decodes staged keys: 4 KiB; with one, 32 KiB (4 to
+        //    32 KiB measured alike on config 4);
+        //  * items of kBatchItemEntries entries: the automatic size fills the
+        //    GPU with one matrix's tiles (down to 1024 entries per item, up
+        //    to whatever one round of tiles needs), short items pay their
+        //    start-up for little work and long ones leave a unit's waves
+        //    waiting for its slowest (1024: 1.521, 1536: 1.481, 2048:
+        //    1.466, 2560: 1.500 ms; 2048 or the one-round size, if larger:
+        //    1.515 ms).  A matrix with more than 8 x 2048 entries per CU
+        //    then has more tiles than a launch of its own would run at once,
+        //    which a batch's unit queues absorb.
+        if (o.stream_tile_waves == 0) oc.stream_tile_waves = cbs::kBatchTileWaves;
+        if (o.stream_tile_kib == 0) oc.stream_tile_kib = nrhs > 1 ? cbs::kBatchMultiTileKib : cbs::kBatchTileKib;
+        if (o.stream_item_nnz == 0) oc.stream_item_nnz = cbs::kBatchItemEntries;
         try_candidate(oc, true, timed);
     }
     (void)hipEventDestroy(e0);

@@ -265,19 +265,20 @@ void cbspmv_batch_destroy(cbspmv_batch* batch);
  * per copy, i.e. what the matrix costs inside a full batch; a plan a batch
  * does not take is timed with nrhs single launches) and returns the
  * fastest, destroying the others; a later candidate replaces an earlier
- * one only when it is at least 5% faster, so list the preferred one first.  *best_ms is that per-copy time.  `candidates`
- * lists (block_size, waves) pairs; waves sets both the blocked schedule's
- * waves per workgroup and the stream schedule's tile waves, 0 keeps both
- * as `base` has them (its auto tile waves by default).
- * NULL = {16,0},{32,0},{16,4},{16,9},{32,4},{32,9}.  A stream-schedule
- * candidate without DENSE blocks whose stream plan has the shape of one
- * already timed (same tile waves, items, slab bytes and staged lines: the
- * block size then changes nothing the stream kernel reads) is not timed;
- * the earlier candidate stands for it.  Options `base` leaves at 0 get a
- * batch shape: items of at least 2048 entries (stream_item_nnz; the
- * automatic size fills the GPU with one matrix and goes down to 1024), and
- * with nrhs > 1 a 4 KiB LDS x cache (stream_tile_kib; several right-hand
- * sides gather x instead of staging it).
+ * one only when it is at least 5% faster, so list the preferred one first.
+ * *best_ms is that per-copy time.  `candidates` lists (block_size, waves)
+ * pairs; waves sets the blocked schedule's waves per workgroup (the
+ * reference's CB_WAVES_PER_BLOCK), 0 keeps `base`'s.
+ * NULL = {16,0},{32,0},{16,4},{16,9},{32,4},{32,9}.
+ * The stream tile shape is the batch's, not a candidate's: a batched launch
+ * runs every unit with its widest tile and largest LDS cache.  Options
+ * `base` leaves at 0 get the batch shape: 8 tile waves, items of 2048
+ * entries, an LDS x cache of 32 KiB (4 KiB with nrhs > 1, whose x is
+ * gathered rather than staged).  A stream-schedule candidate without DENSE
+ * blocks whose stream plan has the shape of one already timed (same tile
+ * waves, items, slab bytes and staged lines: the block size and the blocked
+ * schedule's waves then change nothing the stream kernel reads) is not
+ * timed; the earlier candidate stands for it.
  * Allocates device memory for its test vectors. */
 int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const double* val,
                     const cbspmv_opts* base, const int* candidates, int ncandidates,

@@ -196,15 +196,17 @@ def test_autotune_returns_a_correct_plan(gpu):
     r, c, v = gen.random_coo(m, n, 12, seed=5)
     p, ms = autotune(m, n, r, c, v, nrhs=2, dtype="f32")
     assert ms > 0
-    waves, _ = p.export("stream_tile_shape")
-    assert int(waves) in (4, 9, 16)   # 16: the auto shape of FP32 plans
+    waves, lds = p.export("stream_tile_shape")
+    assert (int(waves), int(lds)) == (8, 4096)   # the batch shape with several right-hand sides
     x = np.random.default_rng(6).uniform(-1, 1, n)
     y = p.spmv(torch.tensor(x, dtype=torch.float32, device=gpu)).cpu().numpy().astype(np.float64)
     assert pyoracle.rel_l2(y, pyoracle.spmv_seq(m, r, c, v, x.astype(np.float32).astype(np.float64))) <= 1e-6
     p.close()
-    # explicit candidates are honoured, and the plan runs in a batch
-    p, ms = autotune(m, n, r, c, v, nrhs=5, candidates=[(32, 9)], dtype="f32")
+    # explicit candidates and base options are honoured, and the plan runs
+    # in a batch
+    p, ms = autotune(m, n, r, c, v, nrhs=5, candidates=[(32, 9)], dtype="f32", tile_waves=9)
     assert int(p.export("stream_tile_shape")[0]) == 9
+    assert p.stats()["gb_m"] == (m + 31) // 32
     X = torch.tensor(np.random.default_rng(7).uniform(-1, 1, (5, n)), dtype=torch.float32, device=gpu)
     Y = torch.zeros((5, m), dtype=torch.float32, device=gpu)
     b = Batch([p])
@@ -223,13 +225,14 @@ def test_autotune_returns_a_correct_plan(gpu):
         assert p.stats()["gb_m"] == (m + first - 1) // first
         p.close()
     # options left open get the batch shape: items of about 2048 entries
-    # (the automatic size here would be about 1024) and, with several
-    # right-hand sides, a 4 KiB LDS cache; the plan is correct in a batch
-    p, _ = autotune(m, n, r, c, v, nrhs=3, candidates=[(16, 16)], dtype="f32")
+    # (the automatic size here would be about 1024), 8 tile waves and a
+    # 32 KiB LDS cache with one right-hand side; the plan is correct in a
+    # batch with several
+    p, _ = autotune(m, n, r, c, v, nrhs=1, candidates=[(16, 16)], dtype="f32")
     items = p.export("stream_items").reshape(-1, 4)
     mean = float((items[:, 1] - items[:, 0]).mean())
     assert 1800 <= mean <= 2600, mean
-    assert int(p.export("stream_tile_shape")[1]) == 4096
+    assert [int(t) for t in p.export("stream_tile_shape")] == [8, 32768]
     X = torch.tensor(np.random.default_rng(8).uniform(-1, 1, (3, n)), dtype=torch.float32, device=gpu)
     Y = torch.zeros((3, m), dtype=torch.float32, device=gpu)
     b = Batch([p])
