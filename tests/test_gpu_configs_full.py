@@ -90,19 +90,24 @@ def test_clustered_reduced_precision(gpu, clustered, dtype, comp):
 # ---- 256-matrix batch in one captured launch ---------------------------------
 
 @pytest.fixture(scope="module")
-def batch256():
+def batch256_mats():
     mats = gen.config4()
     print(f"batch: {len(mats)} matrices, {sum(int(d['row'].size) for d in mats)} nnz", flush=True)
-    plans = [Plan(d["m"], d["n"], d["row"], d["col"], d["val"], dtype="f32") for d in mats]
-    yield mats, plans
-    for p in plans:
-        p.close()
+    return mats
 
 
-def test_batch_256_graph_replay(gpu, batch256):
+# default plans, and plans in the shape the autotuner gives a batch (8 tile
+# waves, 2048-entry items, 32 or 4 KiB LDS caches: two workgroups per CU)
+@pytest.mark.parametrize("shape", ["default", "batch"])
+def test_batch_256_graph_replay(gpu, batch256_mats, shape):
     import torch
 
-    mats, plans = batch256
+    mats = batch256_mats
+    def opts(d):
+        if shape == "default":
+            return {}
+        return dict(tile_waves=8, item_nnz=2048, tile_kib=4 if d["nrhs"] > 1 else 32)
+    plans = [Plan(d["m"], d["n"], d["row"], d["col"], d["val"], dtype="f32", **opts(d)) for d in mats]
     X = [torch.tensor(d["X"], dtype=torch.float32, device=gpu) for d in mats]
     Y = [torch.full((d["nrhs"], d["m"]), float("nan"), dtype=torch.float32, device=gpu) for d in mats]
     eager = [torch.empty_like(y) for y in Y]
@@ -130,7 +135,9 @@ def test_batch_256_graph_replay(gpu, batch256):
             worst = max(worst, pyoracle.rel_l2(ys[k], ref))
             checked += 1
     b.close()
-    print(f"batch: {checked} right-hand sides, worst rel_l2 vs FP64 {worst:.3e}", flush=True)
+    for p in plans:
+        p.close()
+    print(f"batch ({shape}): {checked} right-hand sides, worst rel_l2 vs FP64 {worst:.3e}", flush=True)
     assert checked == sum(d["nrhs"] for d in mats)
     assert worst <= 1e-6, worst
 
