@@ -483,12 +483,13 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
         if (dyk) (void)hipFree(dyk);
         return ms / kReps / per_run;
     };
-    // Builds, times and keeps or destroys one candidate.
-    auto try_candidate = [&](const cbspmv_opts& oc, bool dedupe, std::vector<cbs::StreamPlanShape>& timed) {
+    // Builds, times and keeps or destroys one candidate; `timed` lists the
+    // stream plan shapes timed so far.
+    auto try_candidate = [&](const cbspmv_opts& oc, std::vector<cbs::StreamPlanShape>& timed) {
         cbspmv_plan* p = nullptr;
         rc = cbspmv_build(m, n, nnz, row, col, val, &oc, &p);
         if (rc) return;
-        if (dedupe && p->schedule == CBSPMV_SCHED_STREAM) {
+        if (p->schedule == CBSPMV_SCHED_STREAM) {
             const cbs::StreamPlanShape sh = cbs::stream_plan_shape(p->host);
             if (!sh.dense) {
                 if (std::find(timed.begin(), timed.end(), sh) != timed.end()) {
@@ -545,7 +546,7 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
         if (o.stream_tile_waves == 0) oc.stream_tile_waves = cbs::kBatchTileWaves;
         if (o.stream_tile_kib == 0) oc.stream_tile_kib = nrhs > 1 ? cbs::kBatchMultiTileKib : cbs::kBatchTileKib;
         if (o.stream_item_nnz == 0) oc.stream_item_nnz = cbs::kBatchItemEntries;
-        try_candidate(oc, true, timed);
+        try_candidate(oc, timed);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
