@@ -165,6 +165,20 @@ void pack_canonical(Plan& p) {
 // contiguous run.
 namespace {
 
+// Per-entry cost weights of the automatic item cut (integers, so that item
+// targets stay integral): an entry whose x comes from LDS or is read again
+// in its tile costs kStagedWeight, a random gather the other.  Per-workgroup
+// end times of config 2 with items cut by entry count put stencil entries at
+// 0.66 ns (FP64) / 0.38 ns (FP32) of one CU's time and the power-law half's
+// scattered ones at 2.3 / 1.7 ns.  Swept on config 2 (gathered weight over
+// staged, launch time): FP64 3.5x 135.7 us, 4.5x 136.5, 5.5x 136.2, 7x 145.5
+// (137.0 by entry count); FP32 3.5x 96.7, 4.5x 96.3, 5.5x 94.8, 7x 94.4, 9x
+// 104.0 us (99.7 by entry count).  Heavier weights let a few tiles with many
+// scattered entries run long.
+constexpr int64_t kStagedWeight = 2;
+constexpr int64_t kGatheredWeightF64 = 7;
+constexpr int64_t kGatheredWeightF32 = 12;
+
 // Stream entries in row order.  zero marks an entry that only makes its row a
 // run (value 0, padding key).
 struct Entries {
@@ -197,23 +211,37 @@ StreamShape stream_shape(const Plan& p) {
     return s;
 }
 
-// Entries [b0, e0) of E in items of about `target` entries, cut at row
-// boundaries, an item's rows spanning at most omask + 1 row offsets.  With
-// `split`, a row longer than 4 target is cut into pieces of about target
-// entries that meet in the partial buffer (splits; first_item counts from
-// item_base); without, such a row is one item.
-void cut_items(const Entries& E, int64_t b0, int64_t e0, int64_t target, bool split, uint32_t omask,
+// Entries [b0, e0) of E in items of about `target` cost units, cut at row
+// boundaries, an item's rows spanning at most omask + 1 row offsets.  The
+// cost of entries [a, b) is wp[b] - wp[a] (prefix sums of per-entry
+// weights), or b - a without wp.  With `split`, a row costing more than 4
+// target is cut into pieces of about target (by entry count) that meet in
+// the partial buffer (splits; first_item counts from item_base); without,
+// such a row is one item.
+// Starts of the row runs of E in [b0, e0), then e0.
+std::vector<int64_t> row_runs(const Entries& E, int64_t b0, int64_t e0) {
+    std::vector<int64_t> runs;
+    for (int64_t r = b0; r < e0; ++r)
+        if (r == b0 || E.row[r] != E.row[r - 1]) runs.push_back(r);
+    runs.push_back(e0);
+    return runs;
+}
+
+// `runs`: row_runs(E, b0, e0), so that the repeated cuts of a bisection
+// step over rows rather than entries.
+void cut_items(const Entries& E, const std::vector<int64_t>& runs, int64_t target, bool split, uint32_t omask,
                size_t item_base, std::vector<Piece>& pieces, std::vector<StreamSplit>& splits,
-               uint32_t& partial_slots) {
+               uint32_t& partial_slots, const std::vector<int64_t>* wp = nullptr) {
+    auto cost = [&](int64_t a, int64_t b) { return wp ? (*wp)[b] - (*wp)[a] : b - a; };
     const int64_t split_len = 4 * target;
-    int64_t cur = b0, r = b0;
-    while (r < e0) {
-        int64_t re = r + 1;
-        while (re < e0 && E.row[re] == E.row[r]) ++re;
-        const int64_t len = re - r;
-        if (split && len > split_len) {
+    const int64_t b0 = runs.front(), e0 = runs.back();
+    int64_t cur = b0;
+    for (size_t k = 0; k + 1 < runs.size(); ++k) {
+        const int64_t r = runs[k], re = runs[k + 1];
+        const int64_t len = re - r, len_c = cost(r, re);
+        if (split && len_c > split_len) {
             if (cur < r) pieces.push_back({cur, r, 0, 0, ~0u, ~0u});
-            const int64_t np = (len + target - 1) / target;
+            const int64_t np = (len_c + target - 1) / target;
             StreamSplit sp{};
             sp.row = E.row[r];
             sp.first_item = (uint32_t)(item_base + pieces.size());
@@ -226,11 +254,10 @@ void cut_items(const Entries& E, int64_t b0, int64_t e0, int64_t target, bool sp
                 pieces.push_back({r + len * q / np, r + len * (q + 1) / np, (uint8_t)(q > 0), (uint8_t)(q + 1 < np),
                                   q > 0 ? id : ~0u, q + 1 < np ? id : ~0u});
             cur = re;
-        } else if (r > cur && ((r - cur) + len > target || E.row[re - 1] - E.row[cur] > omask)) {
+        } else if (r > cur && (cost(cur, r) + len_c > target || E.row[re - 1] - E.row[cur] > omask)) {
             pieces.push_back({cur, r, 0, 0, ~0u, ~0u});
             cur = r;
         }
-        r = re;
     }
     if (cur < e0) pieces.push_back({cur, e0, 0, 0, ~0u, ~0u});
 }
@@ -290,8 +317,10 @@ void append_items(Plan& p, const Entries& E, const std::vector<Piece>& pieces, i
 // the most hit ones when more than fit) and the keys.  Resets the stream
 // tables of p.  src receives the entry of E at each padded position.
 void layout_tiles(Plan& p, const StreamShape& sh, const Entries& E, std::vector<uint32_t>& p_col,
-                  std::vector<uint32_t>& p_off, std::vector<uint8_t>& p_zero, std::vector<int64_t>& src) {
+                  std::vector<uint32_t>& p_off, std::vector<uint8_t>& p_zero, std::vector<int64_t>& src,
+                  const std::vector<int64_t>* wp = nullptr) {
     const int64_t total = E.size();
+    const int64_t total_c = wp ? wp->back() : total;   // cost of all entries
     p.items.clear();
     p.item_split.clear();
     p.item_split_first.clear();
@@ -300,11 +329,12 @@ void layout_tiles(Plan& p, const StreamShape& sh, const Entries& E, std::vector<
     p.s_val.clear();
     p_col.clear(); p_off.clear(); p_zero.clear(); src.clear();
     std::vector<Piece> pieces;
+    const std::vector<int64_t> runs = row_runs(E, 0, total);
     auto cut = [&](int64_t target) {
         pieces.clear();
         p.splits.clear();
         p.partial_slots = 0;
-        cut_items(E, 0, total, target, true, sh.omask, 0, pieces, p.splits, p.partial_slots);
+        cut_items(E, runs, target, true, sh.omask, 0, pieces, p.splits, p.partial_slots, wp);
     };
     if (p.o.item_nnz > 0) {
         cut(std::max<int64_t>(64, std::min<int64_t>(p.o.item_nnz, 1 << 22)));
@@ -314,11 +344,14 @@ void layout_tiles(Plan& p, const StreamShape& sh, const Entries& E, std::vector<
         // the cut fits, then bisect back towards the smallest that fits, so
         // that few CUs are left without a tile (30M entries in FP64: 256
         // tiles instead of 249)
+        // (in cost units with weights: at least kMinItemEntries entries of
+        // the average cost)
         const int64_t max_items = (int64_t)p.o.tiles * p.o.tile_items;
-        int64_t lo = std::max<int64_t>(kMinItemEntries, total / max_items + 1), hi = lo;
+        const int64_t min_item = total > 0 ? std::max<int64_t>(1, kMinItemEntries * total_c / total) : kMinItemEntries;
+        int64_t lo = std::max<int64_t>(min_item, total_c / max_items + 1), hi = lo;
         cut(hi);
         bool fits = (int64_t)pieces.size() <= max_items;
-        for (int round = 0; round < 64 && !fits && hi < (1 << 22); ++round) {
+        for (int round = 0; round < 64 && !fits && hi < ((int64_t)1 << 40); ++round) {
             lo = hi + 1;
             hi = hi + hi / 16 + 1;
             cut(hi);
@@ -511,6 +544,50 @@ void build_stream(Plan& p) {
     std::vector<int64_t> src;
     layout_tiles(p, sh, E, p_col, p_off, p_zero, src);
 
+    // ---- items of equal cost: an entry whose x line no other entry of its
+    // tile reads (a random gather: a vector-L1 miss served from L2 or
+    // beyond) costs several times one whose x is staged in LDS or read again
+    // from the L1, so items cut by entry count leave the CUs whose tiles
+    // stage everything idle for much of the launch (config 2 FP64: the
+    // stencil tiles end at ~78 us, the power-law tiles at 132-140 us).  The
+    // automatic cut is redone with each entry weighted as the first layout
+    // placed it (kGatheredWeight for an unstaged entry whose line is read
+    // once in its tile, kStagedWeight for the others) and the tiles' staged
+    // lines chosen again.
+    if (p.o.item_nnz == 0 && stream_panels(p) <= 1) {
+        const int64_t wg = p.o.dtype == CBSPMV_F64 ? kGatheredWeightF64 : kGatheredWeightF32;
+        std::vector<int64_t> wp((size_t)total + 1, kStagedWeight);
+        wp[0] = 0;
+        const size_t NI = p.items.size(), TI = (size_t)p.o.tile_items;
+        const size_t NT = (NI + TI - 1) / TI;
+        int64_t n_single = 0;
+#pragma omp parallel reduction(+ : n_single)
+        {
+            std::vector<std::pair<uint32_t, int64_t>> ln;   // (line, entry of E)
+#pragma omp for schedule(dynamic, 4)
+            for (long long t = 0; t < (long long)NT; ++t) {
+                const int64_t b = p.items[t * TI].e_begin;
+                const int64_t e = p.items[std::min(NI, (size_t)(t + 1) * TI) - 1].e_end;
+                ln.clear();
+                for (int64_t o = b; o < e; ++o)
+                    if (src[o] >= 0 && (p.s_key[o] & kKeyCached) == 0) ln.push_back({p_col[o] / kLineElems, src[o]});
+                std::sort(ln.begin(), ln.end());
+                for (size_t q = 0; q < ln.size(); ++q) {
+                    const bool alone = (q == 0 || ln[q - 1].first != ln[q].first) &&
+                                       (q + 1 == ln.size() || ln[q + 1].first != ln[q].first);
+                    if (alone) {
+                        wp[(size_t)ln[q].second + 1] = wg;
+                        ++n_single;
+                    }
+                }
+            }
+        }
+        if (n_single > 0) {
+            for (int64_t e = 0; e < total; ++e) wp[(size_t)e + 1] += wp[(size_t)e];
+            layout_tiles(p, sh, E, p_col, p_off, p_zero, src, &wp);
+        }
+    }
+
     // ---- column panels: the entries whose x line is not staged in LDS are
     // moved to panel items, one row-ordered stream per panel of columns, so
     // that the waves of one XCD gather x from one panel only (held in that
@@ -571,7 +648,7 @@ void build_stream(Plan& p) {
                 if (Uq.size() > 0) {
                     const int64_t target = std::max<int64_t>(256, (Uq.size() + waves - 1) / waves);
                     pieces.clear();
-                    cut_items(Uq, 0, Uq.size(), target, false, sh.omask, 0, pieces, no_splits, no_slots);
+                    cut_items(Uq, row_runs(Uq, 0, Uq.size()), target, false, sh.omask, 0, pieces, no_splits, no_slots);
                     append_items(p, Uq, pieces, epl, p_col, p_off, p_zero, nullptr);
                 }
                 p.panel_item_ptr[q + 1] = (uint32_t)p.items.size();
