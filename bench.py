@@ -49,6 +49,7 @@ sys.path.insert(0, os.path.join(ROOT, "cb-sparse-dcu_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+EXTENSION_TIMEOUT_S = 600   # each extension workload of the default line (they take 20-80 s)
 
 
 def log(msg):
@@ -586,14 +587,20 @@ def run_extensions(args):
         t0 = time.perf_counter()
         cmd = [sys.executable, os.path.abspath(__file__), "--workload", name, "--steps", str(max(args.steps, 200)),
                "--warmup", str(max(args.warmup, 20))]
-        proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
-        sys.stderr.write(proc.stderr)
-        lines = [ln for ln in proc.stdout.splitlines() if ln.startswith("{")]
-        if proc.returncode == 0 and lines:
-            r = json.loads(lines[-1])
-            ext[name] = {k: r[k] for k in keep if k in r}
-        else:   # reported in the line, never hidden
-            ext[name] = {"error": f"exit status {proc.returncode}", "stderr_tail": proc.stderr[-2000:]}
+        try:
+            proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                                  timeout=EXTENSION_TIMEOUT_S)
+        except subprocess.TimeoutExpired as e:   # reported in the line, never hidden
+            err = e.stderr.decode(errors="replace") if isinstance(e.stderr, bytes) else (e.stderr or "")
+            ext[name] = {"error": f"no result within {EXTENSION_TIMEOUT_S} s", "stderr_tail": err[-2000:]}
+        else:
+            sys.stderr.write(proc.stderr)
+            lines = [ln for ln in proc.stdout.splitlines() if ln.startswith("{")]
+            if proc.returncode == 0 and lines:
+                r = json.loads(lines[-1])
+                ext[name] = {k: r[k] for k in keep if k in r}
+            else:   # reported in the line, never hidden
+                ext[name] = {"error": f"exit status {proc.returncode}", "stderr_tail": proc.stderr[-2000:]}
         ext[name]["seconds"] = round(time.perf_counter() - t0, 1)
         log(f"extension {name}: {ext[name].get('value')} in {ext[name]['seconds']} s")
     return ext
