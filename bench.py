@@ -2,7 +2,7 @@
 
 A step is one SpMV y = A x over the resident plan (the reference protocol,
 cb-spmv-dcu/src/cb-spmv-dcu.hpp:1631-1660: W untimed warm-up launches, then K
-timed launches between events).  Default workload: config 2 (1,048,576^2,
+timed launches between events).  Default workload: stencil_powerlaw_1m (1,048,576^2,
 ~30.2M nnz: 9-point stencil half + power-law half, FP64, 16x16 blocks), the
 largest single-GPU configuration the metric is quoted on.
 
@@ -62,18 +62,15 @@ def algorithmic_bytes(m, n, nnz, dtype):
     return nnz * (s_v + 4) + 4 * (m + 1) + s_x * n + s_x * m
 
 
-def load_workload(config):
+MATRICES = ("poisson_4096", "stencil_powerlaw_1m", "clustered_2m", "powerlaw_512k")
+
+
+def load_workload(name):
     from cbspmv import gen
 
-    if config == 1:
-        return gen.config1()
-    if config == 2:
-        return gen.config2()
-    if config == 3:
-        return gen.config3()
-    if config == 5:
-        return gen.config5()
-    raise SystemExit(f"unknown config {config}")
+    if name not in MATRICES:
+        raise SystemExit(f"unknown matrix {name}")
+    return getattr(gen, name)()
 
 
 def cpu_baseline(d, budget_s):
@@ -224,7 +221,7 @@ def bench_precision(args):
 
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
-    d = gen.config3()
+    d = gen.clustered_2m()
     m, n, nnz = d["m"], d["n"], int(d["row"].size)
     log(f"precision matrix {m}x{n}, nnz {nnz}, generated in {time.perf_counter() - t0:.1f}s")
     threads = min(16, os.cpu_count() or 1)
@@ -249,7 +246,7 @@ def bench_precision(args):
     speed = res["f64"]["us"] / res["f32"]["us"]
     # the same comparison on the 30M-entry matrix of the default line, whose
     # FP32 x (4 MB) fits one XCD's L2 where the FP64 x (8 MB) does not
-    d2 = gen.config2()
+    d2 = gen.stencil_powerlaw_1m()
     m2, n2, nnz2 = d2["m"], d2["n"], int(d2["row"].size)
     cfg2 = {}
     for dtype in ("f64", "f32"):
@@ -263,18 +260,18 @@ def bench_precision(args):
                        "GB_s": round(algorithmic_bytes(m2, n2, nnz2, dtype) / (ms * 1e-3) / 1e9, 1)}
         plan.close()
     cfg2["f32_over_f64"] = round(cfg2["f64"]["us"] / cfg2["f32"]["us"], 3)
-    res["config2_30M"] = cfg2
-    log(f"config 2: f64 {cfg2['f64']['us']} us, f32 {cfg2['f32']['us']} us")
+    res["stencil_powerlaw_1m"] = cfg2
+    log(f"stencil_powerlaw_1m: f64 {cfg2['f64']['us']} us, f32 {cfg2['f32']['us']} us")
     return {"metric": "FP32-storage SpMV throughput / FP64 on the mixed-precision matrix", "value": round(speed, 3),
             "unit": "x", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": res["f32"]["us"] / 1e3, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded generator, cbspmv/gen.py config3)",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded generator, cbspmv/gen.py clustered_2m)",
             "config": {"workload": "clustered_2M_50M_nnz_mixed_precision", "m": m, "n": n, "nnz": nnz},
             "results": res}
 
 
 def bench_decompose(args):
-    """Times sub-matrices of config 2 on their own, each as its own plan run
+    """Times sub-matrices of stencil_powerlaw_1m on their own, each as its own plan run
     by the same kernel: the stencil half, the power-law half's entries near
     the diagonal (|col - row| <= 4096, the columns the generator draws near
     the row) and its uniformly scattered entries, and everything but the
@@ -295,7 +292,7 @@ def bench_decompose(args):
     from cbspmv import gen
 
     dev = torch.device("cuda:0")
-    d = gen.config2()
+    d = gen.stencil_powerlaw_1m()
     m, n, r, c, v = d["m"], d["n"], d["row"], d["col"], d["val"]
     half = m // 2
     far = np.abs(c.astype(np.int64) - r.astype(np.int64)) > 4096
@@ -327,7 +324,7 @@ def bench_decompose(args):
         del x, y
     res["overlap_us"] = round(res["all_but_scattered"]["us"] + res["power_law_scattered"]["us"]
                               - res["whole"]["us"], 2)
-    return {"decomposition": "config2", "dtype": args.dtype, "steps": args.steps, "warmup": args.warmup,
+    return {"decomposition": "stencil_powerlaw_1m", "dtype": args.dtype, "steps": args.steps, "warmup": args.warmup,
             "parts": res}
 
 
@@ -353,7 +350,7 @@ def bench_autograd(args):
 
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
-    d = gen.config5()
+    d = gen.powerlaw_512k()
     m, n, r, c, v = d["m"], d["n"], d["row"], d["col"], d["val"]
     nnz = int(r.size)
     log(f"autograd matrix {m}x{n}, nnz {nnz}, generated in {time.perf_counter() - t0:.1f}s")
@@ -464,7 +461,7 @@ def bench_autograd(args):
             "value": round(ratio, 3), "unit": "x", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": fwd_step + bwd_step,
             "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic (seeded generator, cbspmv/gen.py config5)",
+            "data": "synthetic (seeded generator, cbspmv/gen.py powerlaw_512k)",
             "config": {"workload": "powerlaw_512k_20M_nnz_autograd", "m": m, "n": n, "nnz": nnz}, "results": res}
 
 
@@ -478,7 +475,7 @@ def bench_batched(args):
 
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
-    mats = gen.config4(count=args.batch_count)
+    mats = gen.batch_256(count=args.batch_count)
     log(f"{len(mats)} matrices, {sum(int(d['row'].size) for d in mats)} nnz, generated in "
         f"{time.perf_counter() - t0:.1f}s")
     stream = torch.cuda.current_stream()
@@ -528,7 +525,7 @@ def bench_batched(args):
     return {"metric": "batched SpMV GB/s (sum of B_alg, x and y per right-hand side), hipGraph replay",
             "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": graph, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic (seeded generator, cbspmv/gen.py config4)",
+            "data": "synthetic (seeded generator, cbspmv/gen.py batch_256)",
             "config": {"workload": f"batched_{len(mats)}_matrices_1to8_rhs", "matrices": len(mats),
                        "nnz": sum(int(d["row"].size) for d in mats), "rhs": sum(d["nrhs"] for d in mats)},
             "results": {"graph_replay_ms": round(graph, 4), "eager_batched_ms": round(batched, 4),
@@ -620,7 +617,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--matrix", default="stencil_powerlaw_1m", choices=MATRICES)
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32", "bf16"])
     ap.add_argument("--compensated", action="store_true")
     ap.add_argument("--block-size", type=int, default=16)
@@ -641,7 +638,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (for scaling PMC byte counters)")
     ap.add_argument("--decompose", action="store_true",
-                    help="time sub-matrices of config 2 (stencil half, near and scattered power-law entries) "
+                    help="time sub-matrices of stencil_powerlaw_1m (stencil half, near and scattered power-law entries) "
                          "each with the same kernel, at --dtype")
     ap.add_argument("--calibrate-gather", action="store_true",
                     help="only run 32M random 8-byte gathers from an 8 MiB FP64 vector (index_select), the "
@@ -693,9 +690,9 @@ def main():
 
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
-    d = load_workload(args.config)
+    d = load_workload(args.matrix)
     m, n, nnz = d["m"], d["n"], int(d["row"].size)
-    log(f"config {args.config}: {m}x{n}, nnz {nnz}, generated in {time.perf_counter() - t0:.1f}s")
+    log(f"{args.matrix}: {m}x{n}, nnz {nnz}, generated in {time.perf_counter() - t0:.1f}s")
     plan = cbspmv.Plan(m, n, d["row"], d["col"], d["val"], block_size=args.block_size, dtype=args.dtype,
                        compensated=args.compensated, schedule=args.schedule, item_nnz=args.item_nnz,
                        tile_waves=args.tile_waves, panels=args.panels)
@@ -737,7 +734,7 @@ def main():
     b_alg = algorithmic_bytes(m, n, nnz, args.dtype)
     gbs = b_alg / (ms * 1e-3) / 1e9
     gflops = 2.0 * nnz / (ms * 1e-3) / 1e9
-    workload = (f"config{args.config}_{args.dtype}{'_comp' if args.compensated else ''}_bs{args.block_size}")
+    workload = (f"{args.matrix}_{args.dtype}{'_comp' if args.compensated else ''}_bs{args.block_size}")
     if not no_cpu:
         cpu_gbs = b_alg / t_cpu / 1e9
         host = pyoracle.host_threads()
@@ -783,7 +780,7 @@ def main():
     }
     plan.close()
     del x, y
-    if not args.no_extensions and args.config == 2 and args.dtype == "f64":
+    if not args.no_extensions and args.matrix == "stencil_powerlaw_1m" and args.dtype == "f64":
         torch.cuda.empty_cache()
         out["extensions"] = run_extensions(args)
     print(json.dumps(out), flush=True)

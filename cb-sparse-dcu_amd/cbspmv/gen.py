@@ -1,31 +1,35 @@
 """Seeded synthetic workloads (row-major COO + x) for the five benchmark
-configurations.  Every generator uses numpy's PCG64 (`numpy.random.default_rng`)
+matrices.  Every generator uses numpy's PCG64 (`numpy.random.default_rng`)
 with the stated seed and only vectorised numpy calls, so a given numpy version
 reproduces the same matrices on every host.
 
-  config1  4096^2, per-row nnz ~ Poisson(10) floored at 1, distinct uniform
-           columns, values U[-1,1], x U[-1,1], seed 0.  Within a row, columns
-           keep their draw order.
-  config2  1,048,576^2: rows [0, 2^19) are a clipped 9-point stencil on a
-           1024 x 1024 grid (row r = grid point (r // 1024, r % 1024),
-           neighbours in (di, dj) lexicographic order); rows [2^19, 2^20) have
-           power-law lengths (alpha = 2, max 8192) with 70% of columns uniform
-           within +-4096 of the diagonal (reflected at the matrix edge) and 30%
-           uniform over all columns, no repeats in a row.  Calibration: the
-           literal minimum length 1 gives ~9.2M nnz, far from the ~30M the
-           configuration names, so lengths are floor(x) of a Pareto(alpha)
-           truncated to [x_min, 8192] with x_min = 7, clipped to >= 1, which
-           gives ~30.5M nnz.  Values U[-1,1], x U[-1,1].
-  config3  2,097,152^2, ~50M nnz: random distinct 64-aligned 64 x 64 tiles,
-           each cell present with probability 0.05, plus 4 distinct uniform
-           columns per row; values N(0,1) * 10^U(-3,3) with one scale per row;
-           entries sorted by (row, col).
-  config4  256 square matrices, n log-uniform in [1e3, 2e5], per-row nnz ~
-           Poisson(U[5,50]) floored at 1, banded (+-max(32, 4 lambda)) or
-           uniform columns chosen per matrix, 1-8 FP32 right-hand sides U[-1,1].
-  config5  524,288^2, power-law lengths (alpha = 2.2, max 4096, x_min = 9
-           calibrated to ~20M nnz), 70% of columns within +-2048 of the
-           diagonal and 30% uniform, no repeats; values U[-1,1], x U[-1,1].
+  poisson_4096
+      4096^2, per-row nnz ~ Poisson(10) floored at 1, distinct uniform
+      columns, values U[-1,1], x U[-1,1], seed 0.  Within a row, columns keep
+      their draw order.
+  stencil_powerlaw_1m
+      1,048,576^2: rows [0, 2^19) are a clipped 9-point stencil on a 1024 x
+      1024 grid (row r = grid point (r // 1024, r % 1024), neighbours in
+      (di, dj) lexicographic order); rows [2^19, 2^20) have power-law lengths
+      (alpha = 2, max 8192) with 70% of columns uniform within +-4096 of the
+      diagonal (reflected at the matrix edge) and 30% uniform over all
+      columns, no repeats in a row.  Calibration: the literal minimum length
+      1 gives ~9.2M nnz, far from the ~30M the workload names, so lengths are
+      floor(x) of a Pareto(alpha) truncated to [x_min, 8192] with x_min = 7,
+      clipped to >= 1, which gives ~30.5M nnz.  Values U[-1,1], x U[-1,1].
+  clustered_2m
+      2,097,152^2, ~50M nnz: random distinct 64-aligned 64 x 64 tiles, each
+      cell present with probability 0.05, plus 4 distinct uniform columns per
+      row; values N(0,1) * 10^U(-3,3) with one scale per row; entries sorted
+      by (row, col).
+  batch_256 (one matrix: batch_matrix)
+      256 square matrices, n log-uniform in [1e3, 2e5], per-row nnz ~
+      Poisson(U[5,50]) floored at 1, banded (+-max(32, 4 lambda)) or uniform
+      columns chosen per matrix, 1-8 FP32 right-hand sides U[-1,1].
+  powerlaw_512k
+      524,288^2, power-law lengths (alpha = 2.2, max 4096, x_min = 9
+      calibrated to ~20M nnz), 70% of columns within +-2048 of the diagonal
+      and 30% uniform, no repeats; values U[-1,1], x U[-1,1].
 """
 import numpy as np
 
@@ -77,7 +81,7 @@ def _power_law_block(rng, first_row, nrows, n, alpha, x_min, l_max, near, near_f
     return rows, cols
 
 
-def config1(seed=0):
+def poisson_4096(seed=0):
     rng = np.random.default_rng(seed)
     m = n = 4096
     k = np.maximum(1, rng.poisson(10, size=m))
@@ -86,7 +90,7 @@ def config1(seed=0):
     val = rng.uniform(-1.0, 1.0, size=rows.size)
     x = rng.uniform(-1.0, 1.0, size=n)
     return dict(m=m, n=n, row=rows.astype(np.int32), col=cols.astype(np.int32), val=val, x=x,
-                name="config1_4096_poisson10")
+                name="poisson10_4096")
 
 
 def stencil9(first_grid_rows, grid=1024):
@@ -107,7 +111,7 @@ def stencil9(first_grid_rows, grid=1024):
     return R[keep], C[keep]
 
 
-def config2(seed=MASTER_SEED + 2, x_min=7.0):
+def stencil_powerlaw_1m(seed=MASTER_SEED + 2, x_min=7.0):
     rng = np.random.default_rng(seed)
     m = n = 1 << 20
     half = m // 2
@@ -118,10 +122,10 @@ def config2(seed=MASTER_SEED + 2, x_min=7.0):
     cols = np.concatenate([c1, c2]).astype(np.int32)
     val = rng.uniform(-1.0, 1.0, size=rows.size)
     x = rng.uniform(-1.0, 1.0, size=n)
-    return dict(m=m, n=n, row=rows, col=cols, val=val, x=x, name="config2_1M_stencil_powerlaw")
+    return dict(m=m, n=n, row=rows, col=cols, val=val, x=x, name="stencil_powerlaw_1M")
 
 
-def config3(seed=MASTER_SEED + 3, scale=1):
+def clustered_2m(seed=MASTER_SEED + 3, scale=1):
     """scale > 1 shrinks m, n and the cluster count by that factor (tests)."""
     rng = np.random.default_rng(seed)
     m = n = (1 << 21) // scale
@@ -154,10 +158,10 @@ def config3(seed=MASTER_SEED + 3, scale=1):
     scale_row = 10.0 ** rng.uniform(-3.0, 3.0, size=m)
     val = rng.standard_normal(rows.size) * scale_row[rows]
     x = rng.uniform(-1.0, 1.0, size=n)
-    return dict(m=m, n=n, row=rows, col=cols, val=val, x=x, name="config3_2M_block_clustered")
+    return dict(m=m, n=n, row=rows, col=cols, val=val, x=x, name="block_clustered_2M")
 
 
-def config4_matrix(i, master_seed=MASTER_SEED + 4):
+def batch_matrix(i, master_seed=MASTER_SEED + 4):
     rng = np.random.default_rng([master_seed, i])
     n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(2e5)))))
     lam = rng.uniform(5.0, 50.0)
@@ -182,7 +186,7 @@ def config4_matrix(i, master_seed=MASTER_SEED + 4):
                 nrhs=nrhs, banded=banded, lam=lam)
 
 
-def config4(count=256, master_seed=MASTER_SEED + 4, workers=None):
+def batch_256(count=256, master_seed=MASTER_SEED + 4, workers=None):
     """The matrices are independent (one seed each), so they are drawn by
     `workers` threads (default: up to 16; numpy releases the GIL in the
     bulk draws and sorts); the result does not depend on the worker count."""
@@ -190,14 +194,14 @@ def config4(count=256, master_seed=MASTER_SEED + 4, workers=None):
 
     workers = min(16, os.cpu_count() or 1) if workers is None else int(workers)
     if workers <= 1 or count < 4:
-        return [config4_matrix(i, master_seed) for i in range(count)]
+        return [batch_matrix(i, master_seed) for i in range(count)]
     import concurrent.futures as cf
 
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
-        return list(ex.map(lambda i: config4_matrix(i, master_seed), range(count)))
+        return list(ex.map(lambda i: batch_matrix(i, master_seed), range(count)))
 
 
-def config5(seed=MASTER_SEED + 5, m=1 << 19, x_min=9.0):
+def powerlaw_512k(seed=MASTER_SEED + 5, m=1 << 19, x_min=9.0):
     rng = np.random.default_rng(seed)
     n = m
     rows, cols = _power_law_block(rng, 0, m, n, alpha=2.2, x_min=x_min, l_max=4096,
@@ -205,7 +209,7 @@ def config5(seed=MASTER_SEED + 5, m=1 << 19, x_min=9.0):
     val = rng.uniform(-1.0, 1.0, size=rows.size)
     x = rng.uniform(-1.0, 1.0, size=n)
     return dict(m=m, n=n, row=rows.astype(np.int32), col=cols.astype(np.int32), val=val, x=x,
-                name="config5_512k_powerlaw_fp32")
+                name="powerlaw_512k_fp32")
 
 
 def random_coo(m, n, per_row, seed=0, dup=False, unit=False, empty_rows=0.0):

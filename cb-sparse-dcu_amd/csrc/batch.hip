@@ -35,8 +35,9 @@ constexpr int kBatchTileKib = 32;         // LDS x cache, one right-hand side
 constexpr int kBatchMultiTileKib = 4;     // LDS x cache, several right-hand sides
 // autotuner: a later candidate replaces the one kept only when it is this
 // much faster.  A candidate is timed against copies of itself, which
-// predicts its cost inside a mixed batch only roughly: on the 256 matrices
-// of config 4, when candidates still chose the stream tile waves, keeping
+// predicts its cost inside a mixed batch only roughly: on a batch of 256
+// matrices (1e3 to 2e5 rows, 258M entries in all), when candidates still
+// chose the stream tile waves, keeping
 // every one that timed faster at all picked 9 waves for 67 matrices and
 // made the batch slower (1.63 ms) than 16 waves for all of them (1.56 ms).
 constexpr double kAutotuneMargin = 0.05;
@@ -529,11 +530,11 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
         // ones leaves waves idle or halves the workgroups per CU):
         //  * 8 tile waves with a cache of at most 32 KiB, so that two
         //    workgroups share a CU and a unit's slowest wave holds up only
-        //    half of it (256 matrices of config 4: 1.557 -> 1.466 ms against
+        //    half of it (a batch of 256 matrices, 258M entries: 1.557 -> 1.466 ms against
         //    16 waves; 4 waves 1.475, 6 1.541, 10 1.685 ms);
         //  * with several right-hand sides x is gathered, not staged, so the
         //    cache only decodes staged keys: 4 KiB; with one, 32 KiB (4 to
-        //    32 KiB measured alike on config 4);
+        //    32 KiB measured alike on that batch);
         //  * items of kBatchItemEntries entries: the automatic size fills the
         //    GPU with one matrix's tiles (down to 1024 entries per item, up
         //    to whatever one round of tiles needs), short items pay their

@@ -168,10 +168,11 @@ namespace {
 // Per-entry cost weights of the automatic item cut (integers, so that item
 // targets stay integral): an entry whose x comes from LDS or is read again
 // in its tile costs kStagedWeight, a random gather the other.  Per-workgroup
-// end times of config 2 with items cut by entry count put stencil entries at
-// 0.66 ns (FP64) / 0.38 ns (FP32) of one CU's time and the power-law half's
-// scattered ones at 2.3 / 1.7 ns.  Swept on config 2 (gathered weight over
-// staged, launch time): FP64 3.5x 135.7 us, 4.5x 136.5, 5.5x 136.2, 7x 145.5
+// end times on the 1,048,576^2 stencil / power-law matrix (30.2M entries)
+// with items cut by entry count put stencil entries at 0.66 ns (FP64) /
+// 0.38 ns (FP32) of one CU's time and the power-law half's scattered ones at
+// 2.3 / 1.7 ns.  Swept on the same matrix (gathered weight over staged,
+// launch time): FP64 3.5x 135.7 us, 4.5x 136.5, 5.5x 136.2, 7x 145.5
 // (137.0 by entry count); FP32 3.5x 96.7, 4.5x 96.3, 5.5x 94.8, 7x 94.4, 9x
 // 104.0 us (99.7 by entry count).  Heavier weights let a few tiles with many
 // scattered entries run long.
@@ -548,8 +549,9 @@ void build_stream(Plan& p) {
     // tile reads (a random gather: a vector-L1 miss served from L2 or
     // beyond) costs several times one whose x is staged in LDS or read again
     // from the L1, so items cut by entry count leave the CUs whose tiles
-    // stage everything idle for much of the launch (config 2 FP64: the
-    // stencil tiles end at ~78 us, the power-law tiles at 132-140 us).  The
+    // stage everything idle for much of the launch (the 1,048,576^2
+    // stencil / power-law matrix in FP64: the stencil tiles end at ~78 us,
+    // the power-law tiles at 132-140 us).  The
     // automatic cut is redone with each entry weighted as the first layout
     // placed it (kGatheredWeight for an unstaged entry whose line is read
     // once in its tile, kStagedWeight for the others) and the tiles' staged

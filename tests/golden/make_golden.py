@@ -54,8 +54,8 @@ def fixture_matrices():
     ptr = z["row_ptr"].astype(np.int64)
     rows = np.repeat(np.arange(int(z["m"]), dtype=np.int32), np.diff(ptr))
     mats["heart2"] = (int(z["m"]), int(z["n"]), rows, z["col"].astype(np.int32), np.ones(rows.size))
-    d = gen.config1()
-    mats["config1"] = (d["m"], d["n"], d["row"], d["col"], d["val"])
+    d = gen.poisson_4096()
+    mats["poisson_4096"] = (d["m"], d["n"], d["row"], d["col"], d["val"])
     r, c, v = gen.random_coo(700, 900, 12, seed=101, dup=True)
     mats["random_dup"] = (700, 900, r, c, v)
     r, c, v = gen.random_coo(5000, 5000, 30, seed=102)

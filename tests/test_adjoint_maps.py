@@ -55,8 +55,8 @@ def _cases():
     c = np.concatenate([c[:at], lc, c[at:]])
     v = np.concatenate([v[:at], np.linspace(-1, 1, lc.size), v[at:]])
     yield "long_row", (300, 40000, r, c, v)
-    d = gen.config1()
-    yield "config1", (d["m"], d["n"], d["row"], d["col"], d["val"])
+    d = gen.poisson_4096()
+    yield "poisson_4096", (d["m"], d["n"], d["row"], d["col"], d["val"])
 
 
 CASES = dict(_cases())

@@ -47,8 +47,8 @@ def matrix(name):
             m, n = int(z["m"]), int(z["n"])
             r = np.repeat(np.arange(m, dtype=np.int32), np.diff(ptr))
             c, v = z["col"].astype(np.int32), np.ones(r.size)
-        elif name == "config1":
-            d = gen.config1()
+        elif name == "poisson_4096":
+            d = gen.poisson_4096()
             m, n, r, c, v = d["m"], d["n"], d["row"], d["col"], d["val"]
         elif name == "random_dup":
             m, n = 700, 900

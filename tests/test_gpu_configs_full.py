@@ -34,7 +34,7 @@ pytestmark = [pytest.mark.gpu, pytest.mark.slow, pytest.mark.timeout(900)]
 
 @pytest.fixture(scope="module")
 def clustered():
-    d = gen.config3()
+    d = gen.clustered_2m()
     d["csr"] = pyoracle.CSR(d["m"], d["row"], d["col"], d["val"])
     d["y64"] = d["csr"].spmv(d["x"], THREADS)
     return d
@@ -91,7 +91,7 @@ def test_clustered_reduced_precision(gpu, clustered, dtype, comp):
 
 @pytest.fixture(scope="module")
 def batch256_mats():
-    mats = gen.config4()
+    mats = gen.batch_256()
     print(f"batch: {len(mats)} matrices, {sum(int(d['row'].size) for d in mats)} nnz", flush=True)
     return mats
 
@@ -146,7 +146,7 @@ def test_batch_256_graph_replay(gpu, batch256_mats, shape):
 
 @pytest.fixture(scope="module")
 def powerlaw():
-    d = gen.config5()
+    d = gen.powerlaw_512k()
     r, c = d["row"], d["col"]
     o = np.argsort(c, kind="stable")
     d["csr"] = pyoracle.CSR(d["m"], r, c, d["val"])

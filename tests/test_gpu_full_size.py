@@ -18,14 +18,14 @@ THREADS = 16
 
 
 @pytest.fixture(scope="module")
-def config2():
-    d = gen.config2()
+def stencil_powerlaw():
+    d = gen.stencil_powerlaw_1m()
     return d["m"], d["n"], d["row"], d["col"], d["val"], d["x"]
 
 
 @pytest.fixture(scope="module")
-def plan64(config2):
-    m, n, r, c, v, _ = config2
+def plan64(stencil_powerlaw):
+    m, n, r, c, v, _ = stencil_powerlaw
     p = Plan(m, n, r, c, v)
     yield p
     p.close()
@@ -33,11 +33,11 @@ def plan64(config2):
 
 @pytest.mark.gpu
 @pytest.mark.slow
-def test_full_size_fp64_stream(gpu, config2, plan64):
+def test_full_size_fp64_stream(gpu, stencil_powerlaw, plan64):
     import torch
 
-    m, n, r, c, v, x = config2
-    print(f"config2 stream: m={m} nnz={r.size}", flush=True)
+    m, n, r, c, v, x = stencil_powerlaw
+    print(f"stencil_powerlaw stream: m={m} nnz={r.size}", flush=True)
     y_ref = pyoracle.spmv_csr_omp(m, r, c, v, x, THREADS)
     xd = torch.tensor(x, device=gpu)
     plan64.set_schedule("stream")
@@ -50,11 +50,11 @@ def test_full_size_fp64_stream(gpu, config2, plan64):
 
 @pytest.mark.gpu
 @pytest.mark.slow
-def test_full_size_fp64_blocked(gpu, config2, plan64):
+def test_full_size_fp64_blocked(gpu, stencil_powerlaw, plan64):
     import torch
 
-    m, n, r, c, v, x = config2
-    print("config2 blocked", flush=True)
+    m, n, r, c, v, x = stencil_powerlaw
+    print("stencil_powerlaw blocked", flush=True)
     y_ref = pyoracle.spmv_csr_omp(m, r, c, v, x, THREADS)
     plan64.set_schedule("blocked")
     y = plan64.spmv(torch.tensor(x, device=gpu)).cpu().numpy()
@@ -64,10 +64,10 @@ def test_full_size_fp64_blocked(gpu, config2, plan64):
 
 @pytest.mark.gpu
 @pytest.mark.slow
-def test_full_size_properties(gpu, config2, plan64):
+def test_full_size_properties(gpu, stencil_powerlaw, plan64):
     import torch
 
-    m, n, r, c, v, x = config2
+    m, n, r, c, v, x = stencil_powerlaw
     rng = np.random.default_rng(1)
     x1, x2 = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
     ys = [plan64.spmv(torch.tensor(z, device=gpu)).cpu().numpy() for z in (x1, x2, x1 - 2.0 * x2)]
@@ -81,11 +81,11 @@ def test_full_size_properties(gpu, config2, plan64):
 @pytest.mark.gpu
 @pytest.mark.slow
 @pytest.mark.parametrize("dtype,comp", [("f32", False), ("f32", True), ("bf16", True)])
-def test_full_size_reduced_precision(gpu, config2, dtype, comp):
+def test_full_size_reduced_precision(gpu, stencil_powerlaw, dtype, comp):
     import torch
 
-    m, n, r, c, v, x = config2
-    print(f"config2 {dtype} comp={comp}", flush=True)
+    m, n, r, c, v, x = stencil_powerlaw
+    print(f"stencil_powerlaw {dtype} comp={comp}", flush=True)
     x32 = x.astype(np.float32)
     y_exact = pyoracle.spmv_csr_omp(m, r, c, v, x, THREADS)
     p = Plan(m, n, r, c, v, dtype=dtype, compensated=comp)
@@ -97,17 +97,17 @@ def test_full_size_reduced_precision(gpu, config2, dtype, comp):
 
 @pytest.mark.gpu
 @pytest.mark.slow
-def test_full_size_bs32_both_schedules(gpu, config2):
+def test_full_size_bs32_both_schedules(gpu, stencil_powerlaw):
     """The 32x32 build, where CSR blocks appear (coo2block.h:154-380 selects
     them for blocks of more than 64 entries, here the power-law half's
     near-diagonal blocks), on both schedules: the blocked schedule runs the
     CSR branch of the per-block kernel (cb-spmv-dcu.hpp:881-952)."""
     import torch
 
-    m, n, r, c, v, x = config2
+    m, n, r, c, v, x = stencil_powerlaw
     p = Plan(m, n, r, c, v, block_size=32)
     st = p.stats()
-    print(f"config2 bs32: {st['gb_nnzb']} blocks, COO {st['coo_blocks']} CSR {st['csr_blocks']} "
+    print(f"stencil_powerlaw bs32: {st['gb_nnzb']} blocks, COO {st['coo_blocks']} CSR {st['csr_blocks']} "
           f"DENSE {st['dense_blocks']}", flush=True)
     assert st["csr_blocks"] > 0
     y_ref = pyoracle.spmv_csr_omp(m, r, c, v, x, THREADS)
@@ -116,6 +116,6 @@ def test_full_size_bs32_both_schedules(gpu, config2):
         p.set_schedule(sched)
         y = p.spmv(xd).cpu().numpy()
         err = pyoracle.rel_l2(y, y_ref)
-        print(f"config2 bs32 {sched}: rel_l2 {err:.3e}", flush=True)
+        print(f"stencil_powerlaw bs32 {sched}: rel_l2 {err:.3e}", flush=True)
         assert err <= 1e-12, (sched, err)
     p.close()

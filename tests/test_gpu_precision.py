@@ -35,8 +35,8 @@ def round_values(v, dtype):
     return (h << 16).astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def _config1():
-    d = gen.config1()
+def _poisson_4096():
+    d = gen.poisson_4096()
     return d["m"], d["n"], d["row"], d["col"], d["val"]
 
 
@@ -46,7 +46,7 @@ def _rand(m, n, per_row, seed, **kw):
 
 
 def _scaled(seed=21):
-    """Values spanning 6 decades, one scale per row (config 3's value law)."""
+    """Values spanning 6 decades, one scale per row (the 2,097,152^2 clustered matrix's value law)."""
     m, n, r, c, v = _rand(4000, 4000, 25, seed)
     rng = np.random.default_rng(seed)
     scale = 10.0 ** rng.uniform(-3, 3, m)
@@ -66,7 +66,7 @@ def _long_row():
 
 
 MATRICES = {
-    "config1": _config1,
+    "poisson_4096": _poisson_4096,
     "rand_wide": lambda: _rand(3000, 50000, 12, 22),
     "empty_rows": lambda: _rand(2001, 1777, 9, 23, empty_rows=0.3),
     "scaled": _scaled,

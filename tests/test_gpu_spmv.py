@@ -76,7 +76,7 @@ def _rand(m, n, per_row, **kw):
 
 
 MATRICES = {
-    "config1": lambda: (lambda d: (d["m"], d["n"], d["row"], d["col"], d["val"]))(gen.config1()),
+    "poisson_4096": lambda: (lambda d: (d["m"], d["n"], d["row"], d["col"], d["val"]))(gen.poisson_4096()),
     "rand_dup": lambda: _rand(500, 700, 3, seed=1, dup=True),
     "unit_dup": lambda: _rand(300, 300, 80, seed=2, dup=True, unit=True),
     "unit_sparse": lambda: _rand(3000, 3000, 4, seed=7, unit=True),
@@ -165,7 +165,7 @@ def test_nnz_zero_and_empty(gpu):
 @pytest.mark.gpu
 def test_bench_protocol_matches_oracle(gpu):
     """cbspmv_bench (cb_spmv protocol): first-launch y is returned."""
-    d = gen.config1()
+    d = gen.poisson_4096()
     plan = Plan(d["m"], d["n"], d["row"], d["col"], d["val"])
     y, runtime, gflops = plan.bench(d["x"])
     y_ref = pyoracle.spmv_seq(d["m"], d["row"], d["col"], d["val"], d["x"])

@@ -178,8 +178,8 @@ def test_payload_sums_to_matrix():
 
 def _policy_cases():
     yield "1138_bus", read_mtx(os.path.join(HERE, "golden", "1138_bus.mtx"))
-    d = gen.config1()
-    yield "config1", (d["m"], d["n"], d["row"], d["col"], d["val"])
+    d = gen.poisson_4096()
+    yield "poisson_4096", (d["m"], d["n"], d["row"], d["col"], d["val"])
     for i, (m, n, r, c, v) in enumerate(CASES):
         yield f"case{i}", (m, n, r, c, v)
 

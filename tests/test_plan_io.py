@@ -15,8 +15,8 @@ NAMES = pyoracle.ARRAY_NAMES + ["image", "stream_slab", "stream_key", "stream_va
 
 
 def _plans():
-    d = gen.config1()
-    yield "config1", (d["m"], d["n"], d["row"], d["col"], d["val"]), dict()
+    d = gen.poisson_4096()
+    yield "poisson_4096", (d["m"], d["n"], d["row"], d["col"], d["val"]), dict()
     r, c, v = gen.random_coo(900, 700, 15, seed=3, dup=True)
     yield "dup_lpt_bs32", (900, 700, r, c, v), dict(block_size=32, row_owned=False)
     yield "f32", (900, 700, r, c, v), dict(dtype="f32", compensated=True)
@@ -43,7 +43,7 @@ def test_round_trip(tmp_path, name, mat, opts):
 
 
 def test_rejects_damaged_files(tmp_path):
-    d = gen.config1()
+    d = gen.poisson_4096()
     p = Plan(d["m"], d["n"], d["row"], d["col"], d["val"])
     path = tmp_path / "a.cbplan"
     p.save(path)
@@ -61,7 +61,7 @@ def test_rejects_damaged_files(tmp_path):
 def test_loaded_plan_spmv(tmp_path, gpu):
     import torch
 
-    d = gen.config1()
+    d = gen.poisson_4096()
     p = Plan(d["m"], d["n"], d["row"], d["col"], d["val"])
     path = tmp_path / "c1.cbplan"
     p.save(path)

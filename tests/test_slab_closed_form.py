@@ -219,8 +219,8 @@ def _matrix(name):
         return read_mtx(os.path.join(HERE, "golden", "1138_bus.mtx"))
     if name == "heart2":
         return _heart2()
-    if name == "config1":
-        d = gen.config1()
+    if name == "poisson_4096":
+        d = gen.poisson_4096()
         return d["m"], d["n"], d["row"], d["col"], d["val"]
     if name == "dup_dense":
         # duplicates, dense blocks with a zero, CSR blocks, valued
@@ -257,7 +257,7 @@ OPTION_SETS = [
     dict(bs=8, row_owned=False, pattern_wave=False),
     dict(bs=64, row_owned=True, pattern_wave=True),
 ]
-MATRICES = ["1138_bus", "heart2", "config1", "dup_dense", "unit_dense", "random"]
+MATRICES = ["1138_bus", "heart2", "poisson_4096", "dup_dense", "unit_dense", "random"]
 
 _ARRAYS = ["gb_row_idx", "gb_col_idx", "nnz_per_blk", "type_per_blk", "segment_rows", "segment_begin",
            "segment_end", "segment_direct", "segment_uniform", "pattern_row_ptr", "pattern_col_idx",
@@ -333,7 +333,7 @@ def test_reference_arrays_give_product_image(name, o):
 
 
 @pytest.mark.parametrize("mutate", ["no_pad", "swap_rowcol", "seg_late", "drop_epochs"])
-@pytest.mark.parametrize("name", ["config1", "dup_dense"])
+@pytest.mark.parametrize("name", ["poisson_4096", "dup_dense"])
 def test_closed_form_catches_planted_mistakes(name, mutate):
     """Each planted packer mistake makes the closed form disagree with the
     product's image or its offsets, so the comparison above would catch the

@@ -47,7 +47,7 @@ def decode_stream(plan, dtype="f64"):
 
 
 def _matrices():
-    yield "config1", gen.config1()
+    yield "poisson_4096", gen.poisson_4096()
     for seed, (m, n, pr, dup, unit) in enumerate([(500, 700, 3, True, False), (3000, 3000, 40, False, False),
                                                   (300, 300, 80, True, True), (1001, 777, 6, False, False)]):
         r, c, v = gen.random_coo(m, n, pr, seed=seed, dup=dup, unit=unit, empty_rows=0.2 if seed == 3 else 0.0)
@@ -111,7 +111,7 @@ def test_slab_planes_hold_the_entries(dtype):
     """The device slab stores each chunk of 64 lanes x EPL entries as 16-byte
     lane planes (keys, then values); reading it back plane by plane gives the
     logical entry stream."""
-    d = gen.config1()
+    d = gen.poisson_4096()
     plan = Plan(d["m"], d["n"], d["row"], d["col"], d["val"], dtype=dtype, item_nnz=1000)
     key = plan.export("stream_key")
     val = plan.export("stream_val")
