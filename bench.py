@@ -9,8 +9,10 @@ largest single-GPU configuration the metric is quoted on.
 value      = algorithmic (CSR-equivalent) bytes per SpMV / time per SpMV, GB/s:
              B_alg = nnz (s_v + 4) + 4 (m + 1) + s_x n + s_y m
 roofline   = the same bytes over the average launch duration (HIP events on
-             the launch stream) against 8 TB/s; `traffic` is read from the
-             committed rocprofv3 PMC summary for this workload when present
+             the launch stream) against 8 TB/s; a plan with a scattered-entry
+             split launches two kernels per SpMV, timed together (`kernels`);
+             `traffic` is read from the committed rocprofv3 PMC summary for
+             this workload when present
 cpu_baseline = the reference's own CPU path (check_results_consistency, its
              sequential FP64 SpMV, compiled from the reference sources into
              oracle/_ref), timed on the host on the same matrix; the oracle's
@@ -767,12 +769,15 @@ def main():
         "data": "synthetic (seeded generator, cbspmv/gen.py)",
         "config": {"workload": workload, "m": m, "n": n, "nnz": nnz, "block_size": args.block_size,
                    "schedule": plan.schedule(), "blocks": st["gb_nnzb"], "gather": bool(st["gather_flag"]),
-                   "build_seconds": round(st["build_seconds"], 3)},
+                   "split_entries": st["split_entries"], "build_seconds": round(st["build_seconds"], 3)},
         "gflops": round(gflops, 2),
         "pct_hbm_peak": round(100.0 * gbs / HBM_PEAK_GBS, 2),
         "rel_l2_vs_fp64_oracle": rel,
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": read_traffic(workload),
+                     "kernels": ("stream_spmv (the entries the stream serves) then sweep_spmv (the "
+                                 f"{st['split_entries']} scattered entries, {st['split_blocks']} row blocks), "
+                                 "timed together" if st["split_entries"] else "stream_spmv"),
                      "bytes_per_launch_algorithmic": b_alg,
                      "stream_slab_bytes": st["stream_bytes"], "canonical_slab_bytes": st["slab_bytes"]},
         "cpu_baseline": cpu,

@@ -29,6 +29,7 @@ F64, F32, BF16 = 0, 1, 2
 ACC_NATIVE, ACC_COMPENSATED = 0, 1
 SCHED_AUTO, SCHED_BLOCKED, SCHED_STREAM = 0, 1, 2
 BACKWARD_DIRECT, BACKWARD_FUSED = 0, 1
+SPLIT_AUTO, SPLIT_OFF, SPLIT_ON = 0, 1, 2
 
 
 class CbspmvError(RuntimeError):
@@ -57,6 +58,7 @@ class Opts(ctypes.Structure):
         ("backward_mode", ctypes.c_int),
         ("stream_panels", ctypes.c_int),
         ("stream_tiles", ctypes.c_int),
+        ("stream_split", ctypes.c_int),
     ]
 
 
@@ -77,6 +79,8 @@ class Stats(ctypes.Structure):
         ("slab_bytes", ctypes.c_longlong), ("stream_bytes", ctypes.c_longlong),
         ("restore_entries", ctypes.c_longlong),
         ("build_seconds", ctypes.c_double),
+        ("split_entries", ctypes.c_longlong),
+        ("split_blocks", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -36,6 +36,9 @@ class CBSparse:
     with its values as a stored-order tensor (`values`)."""
 
     def __init__(self, m, n, row, col, val, device=None, requires_grad=False, **opts):
+        # its values are replaced through the full stream, which drops a
+        # scattered-entry split: none is built
+        opts.setdefault("split", "off")
         self.plan = Plan(m, n, row, col, val, **opts)
         self.plan.build_transpose()
         self.m, self.n, self.nnz = self.plan.m, self.plan.n, self.plan.nnz

@@ -34,6 +34,11 @@ _EXPORT_DTYPES = {
     "grad_slot": np.uint32,
     "stream_value_pos": np.int64,
     "slot_of": np.uint32,
+    "split_key": np.uint32,
+    "split_val": np.float64,
+    "split_src": np.uint32,
+    "split_wave_off": np.uint32,
+    "split_shape": np.uint32,
 }
 ARRAY_NAMES = [
     "gb_row_idx", "gb_col_idx", "gb_row_ptr", "nnz_per_blk", "type_per_blk",
@@ -49,7 +54,8 @@ _SCHEDULES = {"auto": _lib.SCHED_AUTO, "blocked": _lib.SCHED_BLOCKED, "stream": 
 
 def make_opts(block_size=16, waves=4, row_owned=True, pattern_wave=True, force_coo64=False,
               dense_sixteenths=15, builder="gather", dtype="f64", compensated=False,
-              schedule="auto", item_nnz=0, tile_waves=0, tile_kib=0, backward="direct", panels=0, tiles=0):
+              schedule="auto", item_nnz=0, tile_waves=0, tile_kib=0, backward="direct", panels=0, tiles=0,
+              split="auto"):
     o = _lib.Opts()
     lib().cbspmv_default_opts(ctypes.byref(o))
     o.block_size = int(block_size)
@@ -68,6 +74,7 @@ def make_opts(block_size=16, waves=4, row_owned=True, pattern_wave=True, force_c
     o.backward_mode = {"fused": _lib.BACKWARD_FUSED, "direct": _lib.BACKWARD_DIRECT}[backward]
     o.stream_panels = int(panels)
     o.stream_tiles = int(tiles)
+    o.stream_split = {"auto": _lib.SPLIT_AUTO, "off": _lib.SPLIT_OFF, "on": _lib.SPLIT_ON}[split]
     return o
 
 

@@ -341,6 +341,7 @@ int cbspmv_set_values_stored(cbspmv_plan* plan, const void* d_vals_stored, void*
         return rc;
     plan->values_replaced = true;
     t->values_replaced = true;
+    cbs::drop_split(*plan);   // the full stream holds the new values
     return CBSPMV_OK;
 }
 
@@ -356,6 +357,7 @@ int cbspmv_set_values(cbspmv_plan* plan, const void* d_vals, void* stream) {
     if ((rc = cbs::ensure_groups(*plan, plan->slot_of, s)) || (rc = cbs::launch_set_values(*plan, d_vals, s)))
         return rc;
     plan->values_replaced = true;
+    cbs::drop_split(*plan);   // the full stream holds the new values
     if (plan->transpose) {
         // A^T's stream index of input entry k is grad_slot[k], so its groups
         // read the same input-order values

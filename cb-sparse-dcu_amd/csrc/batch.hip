@@ -416,6 +416,8 @@ int cbspmv_autotune(int m, int n, int nnz, const int* row, const int* col, const
     cbspmv_opts o;
     if (base) o = *base;
     else cbspmv_default_opts(&o);
+    // the plans run in batches, which read the full stream: no scattered-entry split
+    o.stream_split = CBSPMV_SPLIT_OFF;
     const hipStream_t s = static_cast<hipStream_t>(stream);
     const size_t eb = o.value_dtype == CBSPMV_F64 ? 8 : 4;
     // one x and y block for all candidates, filled with a fixed pattern

@@ -39,6 +39,8 @@ int build_plan_object(const cbspmv_opts& o, int m, int n, int nnz, const int* ro
         o.stream_panels != 8)
         return fail(CBSPMV_ERR_ARG, "stream_panels must be 0, 1, 2, 4 or 8");
     if (o.stream_tiles < 0 || o.stream_tiles > 65536) return fail(CBSPMV_ERR_ARG, "stream_tiles must be in 0..65536");
+    if (o.stream_split < CBSPMV_SPLIT_AUTO || o.stream_split > CBSPMV_SPLIT_ON)
+        return fail(CBSPMV_ERR_ARG, "stream_split must be 0 (auto), 1 (off) or 2 (on)");
     // stream keys hold column + row-in-strip + 2 flag bits in 32 bits
     if ((long long)n > (1ll << (30 - coord_bits(bs))))
         return fail(CBSPMV_ERR_UNSUPPORTED, "n above 2^(30 - log2 block_size) is not supported");
@@ -88,6 +90,17 @@ int build_plan_object(const cbspmv_opts& o, int m, int n, int nnz, const int* ro
         return fail(rc, err);
     }
     p->schedule = canonical ? choose_schedule(*p) : CBSPMV_SCHED_STREAM;
+    if (canonical) {
+        try {
+            rc = build_split(*p, nnz, row, col, val);
+        } catch (const std::bad_alloc&) {
+            rc = fail(CBSPMV_ERR_NOMEM, "out of host memory while splitting the plan");
+        }
+        if (rc != CBSPMV_OK) {
+            cbspmv_destroy(p);
+            return rc;
+        }
+    }
     *out = p;
     return CBSPMV_OK;
 }
@@ -148,6 +161,8 @@ int cbspmv_plan_stats(const cbspmv_plan* plan, cbspmv_stats* s) {
     s->stream_bytes = cbs::stream_bytes_per_spmv(*plan);
     s->restore_entries = (long long)h.block_cols.size();
     s->build_seconds = h.build_seconds;
+    s->split_entries = (long long)plan->sweep.key.size();
+    s->split_blocks = (int)plan->sweep.blocks();
     return CBSPMV_OK;
 }
 
@@ -156,6 +171,28 @@ int cbspmv_export(const cbspmv_plan* plan, const char* name, void* dst, size_t* 
     if (std::strncmp(name, "transpose.", 10) == 0) {
         if (!plan->transpose) return fail(CBSPMV_ERR_STATE, "no transposed plan");
         return cbspmv_export(plan->transpose, name + 10, dst, count);
+    }
+    if (std::strncmp(name, "near.", 5) == 0) {
+        if (!plan->near) return fail(CBSPMV_ERR_STATE, "the plan has no scattered-entry split");
+        return cbspmv_export(plan->near, name + 5, dst, count);
+    }
+    if (std::strncmp(name, "split_", 6) == 0) {
+        const cbs::SweepPart& sw = plan->sweep;
+        const uint32_t shape[3] = {(uint32_t)sw.col_bits, (uint32_t)cbs::kSweepWaves, (uint32_t)sw.max_rows};
+        const void* src = nullptr;
+        size_t n = 0, esz = 4;
+        const char* k = name + 6;
+        if (!std::strcmp(k, "key")) { src = sw.key.data(); n = sw.key.size(); }
+        else if (!std::strcmp(k, "val")) { src = sw.val.data(); n = sw.val.size(); esz = 8; }
+        else if (!std::strcmp(k, "src")) { src = sw.src.data(); n = sw.src.size(); }
+        else if (!std::strcmp(k, "wave_off")) { src = sw.wave_off.data(); n = sw.wave_off.size(); }
+        else if (!std::strcmp(k, "blk_row")) { src = sw.blk_row.data(); n = sw.blk_row.size(); }
+        else if (!std::strcmp(k, "blk_span")) { src = sw.blk_span.data(); n = sw.blk_span.size(); }
+        else if (!std::strcmp(k, "shape")) { src = shape; n = plan->near ? 3 : 0; }
+        else return fail(CBSPMV_ERR_ARG, std::string("unknown export name: ") + name);
+        if (dst && *count >= n && n) std::memcpy(dst, src, n * esz);
+        *count = n;
+        return CBSPMV_OK;
     }
     if (std::strcmp(name, "grad_slot") == 0 || std::strcmp(name, "slot_of") == 0) {
         const std::vector<uint32_t>& v = name[0] == 'g' ? plan->grad_slot : plan->slot_of;

@@ -14,7 +14,8 @@
 // every key's column (or LDS slot) and row, the tiles' line ranges against
 // the line table and the LDS cache, the cut rows' pieces against the items
 // and the partial buffer, the empty rows and the panel tables.  The
-// transposed plan is not saved (cbspmv_build_transpose rebuilds it).
+// transposed plan is not saved (cbspmv_build_transpose rebuilds it), nor a
+// scattered-entry split (a loaded plan runs its full stream).
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -30,7 +31,7 @@ namespace cbs {
 namespace {
 
 constexpr char kMagic[16] = "CBSPMV-PLAN";
-constexpr uint32_t kVersion = 3;
+constexpr uint32_t kVersion = 4;
 constexpr uint32_t kBom = 0x01020304u;
 
 struct Writer {

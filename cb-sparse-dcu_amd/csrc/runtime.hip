@@ -62,7 +62,8 @@ void release_device(DevicePlan& d) {
                          &d.item_split_first, &d.splits, &d.partial, &d.counters, &d.empty, &d.tiles, &d.tile_lines,
                          &d.sv_pos, &d.sv_ptr, &d.sv_src, &d.ss_pos, &d.ss_ptr, &d.ss_src, &d.grad_slot,
                          &d.dvals_stored, &d.in_row, &d.in_col,
-                         &d.panel_items, &d.panel_mask, &d.panel_out})
+                         &d.panel_items, &d.panel_mask, &d.panel_out,
+                         &d.sw_key, &d.sw_val, &d.sw_wave_off, &d.sw_blk_row, &d.sw_blk_span})
         release(*b);
     d.uploaded = false;
     d.sv_groups = 0;
@@ -129,6 +130,21 @@ int upload_plan(cbspmv_plan& p, hipStream_t s) {
         if ((rc = upload_vec(d.panel_items, h.panel_item_ptr, s))) return rc;
         if ((rc = upload_vec(d.panel_mask, h.panel_mask, s))) return rc;
         if ((rc = upload_bytes(d.panel_out, nullptr, (size_t)h.panels * h.m * vec_elem_bytes(h), s))) return rc;
+    }
+    if (p.near) {   // scattered-entry split: the other entries' plan and the sweep arrays
+        if ((rc = upload_plan(*p.near, s))) return rc;
+        const SweepPart& sw = p.sweep;
+        if ((rc = upload_vec(d.sw_key, sw.key, s))) return rc;
+        if (h.o.dtype == CBSPMV_F64) {
+            if ((rc = upload_vec(d.sw_val, sw.val, s))) return rc;
+        } else {
+            const std::vector<float> v32(sw.val.begin(), sw.val.end());
+            if ((rc = upload_vec(d.sw_val, v32, s))) return rc;
+            CBS_HIP(hipStreamSynchronize(s));   // v32 is freed on return
+        }
+        if ((rc = upload_vec(d.sw_wave_off, sw.wave_off, s))) return rc;
+        if ((rc = upload_vec(d.sw_blk_row, sw.blk_row, s))) return rc;
+        if ((rc = upload_vec(d.sw_blk_span, sw.blk_span, s))) return rc;
     }
     CBS_HIP(hipStreamSynchronize(s));
     d.uploaded = true;
@@ -282,7 +298,12 @@ int run_spmv(cbspmv_plan& p, const void* x, void* y, hipStream_t s) {
     }
     int sched = p.forced != CBSPMV_SCHED_AUTO ? p.forced : p.schedule;
     if (p.values_replaced) sched = CBSPMV_SCHED_STREAM;   // only the stream slab takes new values
-    return sched == CBSPMV_SCHED_BLOCKED ? run_blocked(p, x, y, s) : run_stream(p, x, y, s);
+    if (sched == CBSPMV_SCHED_BLOCKED) return run_blocked(p, x, y, s);
+    if (p.near) {   // scattered-entry split: y = the other entries' product, then y += the sweep
+        const int rc = run_spmv(*p.near, x, y, s);
+        return rc != CBSPMV_OK ? rc : run_sweep(p, x, y, s);
+    }
+    return run_stream(p, x, y, s);
 }
 
 }  // namespace cbs
@@ -294,6 +315,7 @@ extern "C" {
 void cbspmv_destroy(cbspmv_plan* plan) {
     if (!plan) return;
     if (plan->transpose) cbspmv_destroy(plan->transpose);
+    if (plan->near) cbspmv_destroy(plan->near);
     cbs::release_device(plan->dev);
     delete plan;
 }

@@ -83,7 +83,29 @@ typedef struct cbspmv_opts {
     int stream_tiles;     /* stream schedule: tiles (workgroups of one round), 0 = the
                              current device's compute units (256, MI355X's count, when
                              no device is visible at build time); 1..65536            */
+    int stream_split;     /* enum cbspmv_split: scattered entries in a sweep kernel  */
 } cbspmv_opts;
+
+/* Scattered-entry split (FP64 / FP32 plans with native accumulation on the
+ * stream schedule, no column panels).  An entry whose x line at most three
+ * entries among the ~nnz/tiles entries around it read (itself included)
+ * costs the stream schedule a vector-L1 miss, and an L2 miss once x outgrows
+ * one XCD's L2.  The split
+ * takes those entries out: the others form a stream plan of their own, and
+ * the scattered ones run after it in a second kernel that adds them to y,
+ * each wave walking its rows' entries in column order so that the waves of
+ * an XCD sweep x together and mostly hit L2.  cbspmv_spmv and cbspmv_bench
+ * run the split; batches, the transposed plan, the backward pass and
+ * cbspmv_save use the plan's full stream.  Replacing values
+ * (cbspmv_set_values, cbspmv_set_values_stored) drops the split: the full
+ * stream takes the new values. */
+enum cbspmv_split {
+    CBSPMV_SPLIT_AUTO = 0, /* split when x holds more than 4 MiB and at most 8
+                              MiB (one to two XCD L2s) and at least 1M entries,
+                              15% of all, are scattered                          */
+    CBSPMV_SPLIT_OFF = 1,
+    CBSPMV_SPLIT_ON = 2    /* split whenever an entry is scattered (tests)       */
+};
 
 /* Builder statistics: the GLOBAL_BLOCK scalars (format.h:63-77) plus counts. */
 typedef struct cbspmv_stats {
@@ -101,6 +123,8 @@ typedef struct cbspmv_stats {
     long long stream_bytes;    /* transcoded stream slab read per SpMV           */
     long long restore_entries; /* |block_cols|                                   */
     double build_seconds;
+    long long split_entries;   /* entries in the sweep kernel (0: no split)     */
+    int split_blocks;          /* its row blocks (workgroups)                   */
 } cbspmv_stats;
 
 /* PROGRAM_METRICS (format.h:185-188): runtime in ms per launch, GFLOPS. */
@@ -138,6 +162,12 @@ int cbspmv_plan_stats(const cbspmv_plan* plan, cbspmv_stats* stats);
  *   stream_empty_rows stream_value_pos(i64) stream_col_bits(i32)
  *   stream_panels (i32: panels, tile item count) stream_panel_items (panel q's
  *   items: [v[q], v[q + 1])) stream_panel_mask(u8)
+ * Scattered-entry split (empty without one): split_key (row in block above
+ *   col_bits | column) split_val(f64, the input values) split_src (input
+ *   entry of each) split_wave_off (block b's wave w: [v[b W + w], v[b W + w +
+ *   1]), W waves) split_blk_row(i32: first row) split_blk_span(i32: rows)
+ *   split_shape (col_bits, W, rows per block at most); "near.<name>" exports
+ *   <name> of the stream plan of the other entries.
  * Returns CBSPMV_ERR_ARG for an unknown name. */
 int cbspmv_export(const cbspmv_plan* plan, const char* name, void* dst, size_t* count);
 /* Byte offset of a slab region in the canonical image ("total" = size). */
