@@ -312,7 +312,7 @@ def bench_decompose(args):
         rs, cs, vs = (r, c, v) if sel is None else (r[sel], c[sel], v[sel])
         if name in moved:
             cs = moved[name].astype(np.int32)
-        plan = cbspmv.Plan(m, n, rs, cs, vs, dtype=args.dtype)
+        plan = cbspmv.Plan(m, n, rs, cs, vs, dtype=args.dtype, split="off")   # the one-stream kernel on each part
         vdt = plan.vector_dtype
         x = torch.tensor(d["x"], dtype=vdt, device=dev)
         y = torch.empty(m, dtype=vdt, device=dev)
@@ -621,6 +621,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--matrix", default="stencil_powerlaw_1m", choices=MATRICES)
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32", "bf16"])
+    ap.add_argument("--split", default="auto", choices=["auto", "off", "on"],
+                    help="the plan's scattered-entry split (auto: the library's choice)")
     ap.add_argument("--compensated", action="store_true")
     ap.add_argument("--block-size", type=int, default=16)
     ap.add_argument("--schedule", default="auto", choices=["auto", "stream", "blocked"])
@@ -697,7 +699,7 @@ def main():
     log(f"{args.matrix}: {m}x{n}, nnz {nnz}, generated in {time.perf_counter() - t0:.1f}s")
     plan = cbspmv.Plan(m, n, d["row"], d["col"], d["val"], block_size=args.block_size, dtype=args.dtype,
                        compensated=args.compensated, schedule=args.schedule, item_nnz=args.item_nnz,
-                       tile_waves=args.tile_waves, panels=args.panels)
+                       tile_waves=args.tile_waves, panels=args.panels, split=args.split)
     st = plan.stats()
     log(f"plan built in {st['build_seconds']:.2f}s: {st['gb_nnzb']} blocks (COO {st['coo_blocks']}, CSR "
         f"{st['csr_blocks']}, DENSE {st['dense_blocks']}), gather {st['gather_flag']}, schedule {plan.schedule()}")
@@ -736,7 +738,8 @@ def main():
     b_alg = algorithmic_bytes(m, n, nnz, args.dtype)
     gbs = b_alg / (ms * 1e-3) / 1e9
     gflops = 2.0 * nnz / (ms * 1e-3) / 1e9
-    workload = (f"{args.matrix}_{args.dtype}{'_comp' if args.compensated else ''}_bs{args.block_size}")
+    workload = (f"{args.matrix}_{args.dtype}{'_comp' if args.compensated else ''}_bs{args.block_size}"
+                f"{'' if args.split == 'auto' else '_split_' + args.split}")
     if not no_cpu:
         cpu_gbs = b_alg / t_cpu / 1e9
         host = pyoracle.host_threads()
