@@ -98,7 +98,9 @@ typedef struct cbspmv_opts {
  * run the split; batches, the transposed plan, the backward pass and
  * cbspmv_save use the plan's full stream.  Replacing values
  * (cbspmv_set_values, cbspmv_set_values_stored) drops the split: the full
- * stream takes the new values. */
+ * stream takes the new values.  A split plan keeps its full stream besides
+ * the other entries' plan and the sweep arrays (about twice the device
+ * memory of an unsplit plan's stream). */
 enum cbspmv_split {
     CBSPMV_SPLIT_AUTO = 0, /* split when x holds more than 4 MiB and at most 8
                               MiB (one to two XCD L2s) and at least 1M entries,
