@@ -5,9 +5,9 @@ FP32-rounded values, <= 1e-6), on matrices with empty rows, rows whose
 entries are all scattered, a row of 9000 scattered entries (a block alone),
 n of 2^25 (128-row blocks), and an all-scattered matrix (empty near plan).
 The sweep sums every row in a fixed order: repeated launches are identical.
-Replacing values drops the split and the full stream carries the new values;
-a batch of split plans runs their full streams; a captured graph replays the
-two launches."""
+Replacing values (input or stored order) drops the split and the full
+stream carries the new values; a batch of split plans runs their full
+streams; a captured graph replays the two launches."""
 import numpy as np
 import pytest
 
@@ -95,6 +95,21 @@ def test_split_matches_unsplit_and_values_drop_it(gpu):
     assert pyoracle.rel_l2(y.cpu().numpy(), pyoracle.spmv_seq(m, r, c, v2, x)) <= 1e-12
     p.close()
     q.close()
+    # values in stored order (the transposed plan's) drop the split as well
+    p = Plan(m, n, r, c, v, split="on")
+    p.build_transpose()
+    p.spmv(xd)
+    assert p.stats()["split_entries"] > 0
+    slot = p.export("grad_slot").astype(np.int64)
+    v3 = np.random.default_rng(4).uniform(-1, 1, v.size)
+    stored = np.zeros(p.stored_entries())
+    np.add.at(stored, slot, v3)
+    p.set_values_stored(torch.tensor(stored, device=gpu))
+    assert p.stats()["split_entries"] == 0
+    y = p.spmv(xd)
+    torch.cuda.synchronize()
+    assert pyoracle.rel_l2(y.cpu().numpy(), pyoracle.spmv_seq(m, r, c, v3, x)) <= 1e-12
+    p.close()
 
 
 @pytest.mark.gpu

@@ -128,6 +128,17 @@ def test_split_policy():
         p.close()
     with pytest.raises(KeyError):
         Plan(m, n, row, col, val, split="sometimes")
+    # the C ABI rejects an option value outside the enum
+    import ctypes
+    from cbspmv import _lib
+    from cbspmv.plan import make_opts
+    o = make_opts()
+    o.stream_split = 3
+    h = ctypes.c_void_p()
+    rc = _lib.lib().cbspmv_build(m, n, row.size, row.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                 col.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                 val.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(o), ctypes.byref(h))
+    assert rc == _lib.ERR_ARG and not h.value
 
 
 def test_split_is_not_saved(tmp_path):
