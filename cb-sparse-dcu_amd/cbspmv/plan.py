@@ -183,7 +183,13 @@ class Plan:
         check(lib().cbspmv_build_transpose(self._h))
 
     def stored_entries(self):
-        return int(lib().cbspmv_stored_entries(self._h))
+        # fixed once the transposed plan exists (-1 before): asked once then
+        n = getattr(self, "_stored", None)
+        if n is None:
+            n = int(lib().cbspmv_stored_entries(self._h))
+            if n >= 0:
+                self._stored = n
+        return n
 
     def spmv_transpose(self, g, dx=None, stream=None):
         """dx = A^T g on the device."""
