@@ -81,21 +81,20 @@ std::vector<uint8_t> classify(int nnz, const int* row, const int* col, int tiles
 }
 
 // The sweep layout of the scattered entries fe (input indices, row-major).
+// Block and wave boundaries are found in one pass; the waves' entries are
+// then ordered block by block in parallel, each into its own output range.
 SweepPart lay_out(const std::vector<uint32_t>& fe, const int* row, const int* col, const double* val, int col_bits,
                   int max_rows) {
     SweepPart sw;
     sw.col_bits = col_bits;
     sw.max_rows = max_rows;
     const int64_t nf = (int64_t)fe.size();
-    sw.key.reserve(fe.size());
-    sw.val.reserve(fe.size());
-    sw.src.reserve(fe.size());
     auto next_row = [&](int64_t k) {   // first entry after k's row
         const int r = row[fe[k]];
         while (k < nf && row[fe[k]] == r) ++k;
         return k;
     };
-    std::vector<uint32_t> grp;
+    std::vector<int64_t> cuts;   // per block: kSweepWaves + 1 boundaries into fe (shared between blocks)
     for (int64_t i = 0; i < nf;) {
         const int r0 = row[fe[i]];
         int64_t j = i;
@@ -104,18 +103,36 @@ SweepPart lay_out(const std::vector<uint32_t>& fe, const int* row, const int* co
             if (j > i && k - i > kSweepBlockEntries) break;
             j = k;
         }
-        // block: entries [i, j), rows [r0, row of entry j - 1]
+        // block: entries [i, j), rows [r0, row of entry j - 1]; its waves cut
+        // at row boundaries near equal shares of its entries
         sw.blk_row.push_back(r0);
         sw.blk_span.push_back(row[fe[j - 1]] - r0 + 1);
         int64_t a = i;
         for (int w = 0; w < kSweepWaves; ++w) {
-            sw.wave_off.push_back((uint32_t)sw.key.size());
+            cuts.push_back(a);
             int64_t b = j;
-            if (w + 1 < kSweepWaves) {   // about (w + 1) / W of the block's entries, at a row boundary
+            if (w + 1 < kSweepWaves) {
                 b = std::max(a, i + (j - i) * (w + 1) / kSweepWaves);
                 if (b > i && b < j && row[fe[b]] == row[fe[b - 1]]) b = next_row(b);
                 b = std::min(b, j);
             }
+            a = b;
+        }
+        i = j;
+    }
+    cuts.push_back(nf);
+    const long long waves = (long long)cuts.size() - 1;
+    sw.wave_off.assign(cuts.begin(), cuts.end());
+    sw.key.resize((size_t)nf);
+    sw.val.resize((size_t)nf);
+    sw.src.resize((size_t)nf);
+#pragma omp parallel
+    {
+        std::vector<uint32_t> grp;
+#pragma omp for schedule(dynamic, 16)
+        for (long long w = 0; w < waves; ++w) {
+            const int64_t a = cuts[w], b = cuts[w + 1];
+            const int r0 = sw.blk_row[(size_t)(w / kSweepWaves)];
             grp.assign(fe.begin() + a, fe.begin() + b);
             std::stable_sort(grp.begin(), grp.end(), [&](uint32_t x, uint32_t y) {
                 return col[x] != col[y] ? col[x] < col[y] : row[x] < row[y];
@@ -123,16 +140,14 @@ SweepPart lay_out(const std::vector<uint32_t>& fe, const int* row, const int* co
             for (size_t q = 0; q < grp.size(); q += 64)
                 std::stable_sort(grp.begin() + q, grp.begin() + std::min(grp.size(), q + 64),
                                  [&](uint32_t x, uint32_t y) { return row[x] < row[y]; });
-            for (uint32_t e : grp) {
-                sw.key.push_back(((uint32_t)(row[e] - r0) << col_bits) | (uint32_t)col[e]);
-                sw.val.push_back(val[e]);
-                sw.src.push_back(e);
+            for (size_t t = 0; t < grp.size(); ++t) {
+                const uint32_t e = grp[t];
+                sw.key[(size_t)a + t] = ((uint32_t)(row[e] - r0) << col_bits) | (uint32_t)col[e];
+                sw.val[(size_t)a + t] = val[e];
+                sw.src[(size_t)a + t] = e;
             }
-            a = b;
         }
-        i = j;
     }
-    sw.wave_off.push_back((uint32_t)sw.key.size());
     return sw;
 }
 
