@@ -224,4 +224,35 @@ bool stream_entry_slots(const Plan& p, std::vector<uint32_t>& slot);
 // Byte offset in s_slab of the value of padded stream entry e.
 size_t stream_value_offset(const Plan& p, uint32_t item, uint32_t e);
 
+// Scattered entries of a split plan, as the sweep kernel reads them: row
+// blocks of at most kSweepMaxRows rows (the LDS row tile), each run by one
+// workgroup of kSweepWaves waves; a block's rows are cut into kSweepWaves
+// runs of about equal entry counts, one per wave; a wave's entries are in
+// column order (then row, then input order), and every 64 of them from the
+// wave's first are reordered by row (stable), so that one segmented scan
+// sums each 64-entry chunk per row.
+constexpr int kSweepWaves = 4;
+constexpr int kSweepMaxRows = 1024;
+struct SweepPart {
+    int col_bits = 0;                 // key: column below, row in block above
+    int max_rows = 0;                 // rows per block at most
+    std::vector<uint32_t> key;
+    std::vector<double> val;          // input values (the device copy is in the plan dtype)
+    std::vector<uint32_t> src;        // input entry of each
+    std::vector<uint32_t> wave_off;   // blocks * kSweepWaves + 1
+    std::vector<int32_t> blk_row, blk_span;
+    size_t blocks() const { return blk_row.size(); }
+};
+
+// Scattered-entry split, host layout (split_layout.cpp).  1 for the entries
+// of a row-major COO whose x line at most kScatteredReads entries of their
+// group (about nnz / tiles entries, cut at row boundaries) read; line_elems
+// x elements per 128-byte line.
+constexpr size_t kScatteredReads = 3;
+std::vector<uint8_t> scattered_entries(int nnz, const int* row, const int* col, int tiles, uint32_t line_elems);
+// The sweep layout of the entries fe (input indices, row-major order) of
+// a matrix whose columns fit col_bits, blocks of at most max_rows rows.
+SweepPart sweep_layout(const std::vector<uint32_t>& fe, const int* row, const int* col, const double* val,
+                       int col_bits, int max_rows);
+
 }  // namespace cbs

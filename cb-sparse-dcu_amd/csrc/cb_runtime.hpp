@@ -46,25 +46,6 @@ struct DevicePlan {
     DeviceBuf sw_key, sw_val, sw_wave_off, sw_blk_row, sw_blk_span;
 };
 
-// Scattered entries of a split plan, as the sweep kernel reads them: row
-// blocks of at most kSweepMaxRows rows (the LDS row tile), each run by one
-// workgroup of kSweepWaves waves; a block's rows are cut into kSweepWaves
-// runs of about equal entry counts, one per wave; a wave's entries are in
-// column order (then row, then input order), and every 64 of them from the
-// wave's first are reordered by row (stable), so that one segmented scan
-// sums each 64-entry chunk per row.
-constexpr int kSweepWaves = 4;
-constexpr int kSweepMaxRows = 1024;
-struct SweepPart {
-    int col_bits = 0;                 // key: column below, row in block above
-    int max_rows = 0;                 // rows per block at most
-    std::vector<uint32_t> key;
-    std::vector<double> val;          // input values (the device copy is in the plan dtype)
-    std::vector<uint32_t> src;        // input entry of each
-    std::vector<uint32_t> wave_off;   // blocks * kSweepWaves + 1
-    std::vector<int32_t> blk_row, blk_span;
-    size_t blocks() const { return blk_row.size(); }
-};
 
 }  // namespace cbs
 
@@ -121,8 +102,9 @@ bool stream_vectors_fit(const Plan& h);
 int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s, const void* xrow = nullptr,
                void* dvals = nullptr);
 int run_spmv(cbspmv_plan& p, const void* x, void* y, hipStream_t s);
-// Scattered-entry split (split.cpp): built after the plan when its options
-// and the matrix call for it; dropped when values are replaced.
+// Scattered-entry split (split.cpp; its layout in split_layout.cpp): built
+// after the plan when its options and the matrix call for it; dropped when
+// values are replaced.
 int build_split(cbspmv_plan& p, int nnz, const int* row, const int* col, const double* val);
 void drop_split(cbspmv_plan& p);
 // The sweep kernel: y += the scattered entries' products (runtime.hip).

@@ -1,5 +1,6 @@
 // Host-only driver for the AddressSanitizer / UndefinedBehaviorSanitizer
-// build of the CB builder and the packers (builder.cpp, packer.cpp), run by
+// build of the CB builder, the packers and the scattered-entry split's host
+// layout (builder.cpp, packer.cpp, split_layout.cpp), run by
 // tests/test_sanitize_builder.py.  It feeds them malformed and edge-case COO
 // streams (the reference builder has no error path for these and its
 // behaviour on them is undefined, coo2block.h:77-121) and a seeded fuzz of
@@ -74,6 +75,38 @@ int run(const Coo& a, const Opt& o) {
     }
     std::vector<uint32_t> slot;
     expect(cbs::stream_entry_slots(p, slot), "stream entries hold the input entries");
+    // the scattered-entry split's host layout of the same entries: every
+    // scattered entry once, decoding back to its row and column
+    int cb = 1;
+    while ((1ll << cb) < (long long)a.n) ++cb;
+    const int max_rows = cb >= 26 ? 0 : std::min(cbs::kSweepMaxRows, 1 << (32 - cb));
+    for (int tiles : {1, 7, 256}) {
+        const std::vector<uint8_t> far = cbs::scattered_entries(nnz, a.row.data(), a.col.data(), tiles, 16);
+        std::vector<uint32_t> fe;
+        for (int e = 0; e < nnz; ++e)
+            if (far[e]) fe.push_back((uint32_t)e);
+        if (fe.empty() || max_rows < 64) continue;
+        const cbs::SweepPart sw = cbs::sweep_layout(fe, a.row.data(), a.col.data(), a.val.data(), cb, max_rows);
+        const size_t nb = sw.blocks();
+        expect(sw.key.size() == fe.size() && sw.src.size() == fe.size() && sw.val.size() == fe.size(), "split sizes");
+        expect(sw.wave_off.size() == nb * cbs::kSweepWaves + 1 && sw.wave_off.back() == fe.size(), "split wave table");
+        std::vector<uint32_t> seen(sw.src);
+        std::sort(seen.begin(), seen.end());
+        expect(seen == fe, "split holds each scattered entry once");
+        for (size_t w = 0; w + 1 < sw.wave_off.size(); ++w) {
+            const size_t bl = w / cbs::kSweepWaves;
+            for (uint32_t k = sw.wave_off[w]; k < sw.wave_off[w + 1]; ++k) {
+                const uint32_t e = sw.src[k];
+                const int r = sw.blk_row[bl] + (int)(sw.key[k] >> cb);
+                const int c = (int)(sw.key[k] & ((1u << cb) - 1u));
+                if (r != a.row[e] || c != a.col[e] || (int)(sw.key[k] >> cb) >= sw.blk_span[bl] ||
+                    sw.blk_span[bl] > max_rows) {
+                    expect(false, "split entry decodes to its input entry");
+                    return rc;
+                }
+            }
+        }
+    }
     return rc;
 }
 

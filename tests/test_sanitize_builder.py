@@ -1,5 +1,6 @@
-"""The CB builder and the packers (builder.cpp, packer.cpp) compiled on the
-host with AddressSanitizer and UndefinedBehaviorSanitizer, run over
+"""The CB builder, the packers and the scattered-entry split's host layout
+(builder.cpp, packer.cpp, split_layout.cpp) compiled on the host with
+AddressSanitizer and UndefinedBehaviorSanitizer, run over
 malformed COO streams (out-of-range and negative indices, rows out of
 order), edge shapes (nnz = 0, m = 0, n = 0, one entry in a partial block,
 1000 duplicates of one entry, a 40,000-entry row, a full dense matrix with
@@ -25,7 +26,8 @@ def test_builder_and_packers_under_asan_ubsan(tmp_path):
            "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
            f"-I{os.path.join(ROOT, 'include')}", f"-I{SRC}",
            os.path.join(ROOT, "tests", "cpp", "sanitize_builder.cpp"),
-           os.path.join(SRC, "builder.cpp"), os.path.join(SRC, "packer.cpp"), "-o", str(exe)]
+           os.path.join(SRC, "builder.cpp"), os.path.join(SRC, "packer.cpp"), os.path.join(SRC, "split_layout.cpp"),
+           "-o", str(exe)]
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", UBSAN_OPTIONS="print_stacktrace=1",
                OMP_NUM_THREADS="4")
