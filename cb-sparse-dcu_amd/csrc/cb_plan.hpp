@@ -232,7 +232,7 @@ size_t stream_value_offset(const Plan& p, uint32_t item, uint32_t e);
 // wave's first are reordered by row (stable), so that one segmented scan
 // sums each 64-entry chunk per row.
 constexpr int kSweepWaves = 4;
-constexpr int kSweepMaxRows = 1024;
+constexpr int kSweepMaxRows = 2048;
 struct SweepPart {
     int col_bits = 0;                 // key: column below, row in block above
     int max_rows = 0;                 // rows per block at most

@@ -25,15 +25,15 @@
 namespace cbs {
 namespace {
 
-// Automatic split: x larger than one XCD's L2 but at most twice it (a
-// sweep's window of x has to stay in L2 while the waves drift apart), and
-// enough scattered entries for a second launch to pay.  Measured (per SpMV,
-// unsplit -> split): the 1,048,576^2 stencil / power-law matrix, FP64 (x 8
-// MiB, 7.7M of 30.2M entries scattered), 135.9 -> 127.3 us; the 2,097,152^2
-// block-clustered matrix, FP32 (x 8 MiB, 8.1M of 50M), 154.0 -> 153.6 us,
-// and FP64 (x 16 MiB) 217.8 -> 233.2 us, which the x bound leaves unsplit.
+// Automatic split: x larger than one XCD's L2 (below that the gathers hit
+// L2 anyway and the split only adds a launch), at most 16 MiB (the largest
+// x measured), and enough scattered entries for a second launch to pay.
+// Measured (per SpMV, unsplit -> split): the 1,048,576^2 stencil /
+// power-law matrix, FP64 (x 8 MiB, 7.7M of 30.2M entries scattered), 136.1
+// -> 126.7 us; the 2,097,152^2 block-clustered matrix, FP64 (x 16 MiB, 8.2M
+// of 50M), 218.3 -> 196.2 us.
 constexpr size_t kSplitMinXBytes = 4u << 20;
-constexpr size_t kSplitMaxXBytes = 8u << 20;
+constexpr size_t kSplitMaxXBytes = 16u << 20;
 constexpr int64_t kSplitMinEntries = 1 << 20;
 constexpr double kSplitMinShare = 0.15;
 }  // namespace

@@ -102,9 +102,9 @@ typedef struct cbspmv_opts {
  * the other entries' plan and the sweep arrays (about twice the device
  * memory of an unsplit plan's stream). */
 enum cbspmv_split {
-    CBSPMV_SPLIT_AUTO = 0, /* split when x holds more than 4 MiB and at most 8
-                              MiB (one to two XCD L2s) and at least 1M entries,
-                              15% of all, are scattered                          */
+    CBSPMV_SPLIT_AUTO = 0, /* split when x holds more than 4 MiB (one XCD's L2)
+                              and at most 16 MiB, and at least 1M entries, 15%
+                              of all, are scattered                              */
     CBSPMV_SPLIT_OFF = 1,
     CBSPMV_SPLIT_ON = 2    /* split whenever an entry is scattered (tests)       */
 };
