@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <string>
 #include <vector>
@@ -243,6 +244,15 @@ struct SweepPart {
     std::vector<int32_t> blk_row, blk_span;
     size_t blocks() const { return blk_row.size(); }
 };
+
+// The BF16 bits of v (through FP32, round to nearest even; NaNs stay NaN).
+inline uint16_t bf16_bits(double v) {
+    const float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
 
 // Scattered-entry split, host layout (split_layout.cpp).  1 for the entries
 // of a row-major COO whose x line at most kScatteredReads entries of their

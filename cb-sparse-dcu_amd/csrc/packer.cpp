@@ -689,12 +689,7 @@ void build_stream(Plan& p) {
                         const float f = (float)p.s_val[e];
                         std::memcpy(vdst, &f, 4);
                     } else {   // BF16, round to nearest even
-                        const float f = (float)p.s_val[e];
-                        uint32_t u;
-                        std::memcpy(&u, &f, 4);
-                        uint16_t h;
-                        if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) h = (uint16_t)((u >> 16) | 0x40);
-                        else h = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+                        const uint16_t h = bf16_bits(p.s_val[e]);
                         std::memcpy(vdst, &h, 2);
                     }
                 }

@@ -137,10 +137,15 @@ int upload_plan(cbspmv_plan& p, hipStream_t s) {
         if ((rc = upload_vec(d.sw_key, sw.key, s))) return rc;
         if (h.o.dtype == CBSPMV_F64) {
             if ((rc = upload_vec(d.sw_val, sw.val, s))) return rc;
-        } else {
+        } else if (h.o.dtype == CBSPMV_F32) {
             const std::vector<float> v32(sw.val.begin(), sw.val.end());
             if ((rc = upload_vec(d.sw_val, v32, s))) return rc;
             CBS_HIP(hipStreamSynchronize(s));   // v32 is freed on return
+        } else {   // BF16: rounded as the stream stores them
+            std::vector<uint16_t> v16(sw.val.size());
+            for (size_t i = 0; i < v16.size(); ++i) v16[i] = bf16_bits(sw.val[i]);
+            if ((rc = upload_vec(d.sw_val, v16, s))) return rc;
+            CBS_HIP(hipStreamSynchronize(s));   // v16 is freed on return
         }
         if ((rc = upload_vec(d.sw_wave_off, sw.wave_off, s))) return rc;
         if ((rc = upload_vec(d.sw_blk_row, sw.blk_row, s))) return rc;

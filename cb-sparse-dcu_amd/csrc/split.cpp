@@ -42,7 +42,7 @@ int build_split(cbspmv_plan& p, int nnz, const int* row, const int* col, const d
     const Plan& h = p.host;
     const int mode = p.opts.stream_split;
     if (mode == CBSPMV_SPLIT_OFF || nnz == 0 || h.m == 0) return CBSPMV_OK;
-    if ((h.o.dtype != CBSPMV_F64 && h.o.dtype != CBSPMV_F32) || h.o.accum != CBSPMV_ACC_NATIVE || h.o.panels > 1 ||
+    if (h.o.accum != CBSPMV_ACC_NATIVE || h.o.panels > 1 ||
         p.schedule != CBSPMV_SCHED_STREAM)
         return CBSPMV_OK;
     const size_t xb = vec_elem_bytes(h);

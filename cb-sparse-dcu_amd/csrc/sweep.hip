@@ -29,9 +29,18 @@ namespace dev {
 constexpr int kSweepThreads = kSweepWaves * kWave;
 constexpr int kSweepChunks = 1;   // 64-entry chunks per wave per step
 
-template <typename A>
+// A stored value as the accumulation type: FP64 and FP32 as they are, BF16
+// (its 16 bits) widened to FP32.
+template <typename A, typename V>
+__device__ __forceinline__ A sweep_value(const V* v, uint32_t k) { return A(gptr(v)[k]); }
+template <>
+__device__ __forceinline__ float sweep_value<float, uint16_t>(const uint16_t* v, uint32_t k) {
+    return __uint_as_float((uint32_t)gptr(v)[k] << 16);
+}
+
+template <typename A, typename V>
 __global__ __launch_bounds__(kSweepThreads) void sweep_spmv(const A* __restrict__ x, const uint32_t* __restrict__ key,
-                                                            const A* __restrict__ val,
+                                                            const V* __restrict__ val,
                                                             const uint32_t* __restrict__ wave_off,
                                                             const int32_t* __restrict__ blk_row,
                                                             const int32_t* __restrict__ blk_span, uint32_t col_bits,
@@ -49,7 +58,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_spmv(const A* __restrict_
     for (int u = 0; u < U; ++u) {
         const uint32_t k = e0 + u * kWave + lane;
         kn[u] = k < e1 ? gptr(key)[k] : 0u;
-        vn[u] = k < e1 ? gptr(val)[k] : A(0);
+        vn[u] = k < e1 ? sweep_value<A>(val, k) : A(0);
     }
     __syncthreads();
     for (uint32_t k0 = e0; k0 < e1; k0 += U * kWave) {
@@ -67,7 +76,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_spmv(const A* __restrict_
         for (int u = 0; u < U; ++u) {
             const uint32_t k = k1 + u * kWave + lane;
             kn[u] = k < e1 ? gptr(key)[k] : 0u;
-            vn[u] = k < e1 ? gptr(val)[k] : A(0);
+            vn[u] = k < e1 ? sweep_value<A>(val, k) : A(0);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -97,12 +106,16 @@ int run_sweep(cbspmv_plan& p, const void* x, void* y, hipStream_t s) {
     const auto* br = static_cast<const int32_t*>(d.sw_blk_row.ptr);
     const auto* bs = static_cast<const int32_t*>(d.sw_blk_span.ptr);
     if (p.host.o.dtype == CBSPMV_F64)
-        hipLaunchKernelGGL(dev::sweep_spmv<double>, dim3(blocks), dim3(dev::kSweepThreads), 0, s,
+        hipLaunchKernelGGL((dev::sweep_spmv<double, double>), dim3(blocks), dim3(dev::kSweepThreads), 0, s,
                            static_cast<const double*>(x), key, static_cast<const double*>(d.sw_val.ptr), wo, br, bs,
                            (uint32_t)sw.col_bits, static_cast<double*>(y));
-    else
-        hipLaunchKernelGGL(dev::sweep_spmv<float>, dim3(blocks), dim3(dev::kSweepThreads), 0, s,
+    else if (p.host.o.dtype == CBSPMV_F32)
+        hipLaunchKernelGGL((dev::sweep_spmv<float, float>), dim3(blocks), dim3(dev::kSweepThreads), 0, s,
                            static_cast<const float*>(x), key, static_cast<const float*>(d.sw_val.ptr), wo, br, bs,
+                           (uint32_t)sw.col_bits, static_cast<float*>(y));
+    else   // BF16 values, FP32 vectors
+        hipLaunchKernelGGL((dev::sweep_spmv<float, uint16_t>), dim3(blocks), dim3(dev::kSweepThreads), 0, s,
+                           static_cast<const float*>(x), key, static_cast<const uint16_t*>(d.sw_val.ptr), wo, br, bs,
                            (uint32_t)sw.col_bits, static_cast<float*>(y));
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? CBSPMV_OK : hip_fail(e, "sweep launch");

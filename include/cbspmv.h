@@ -86,8 +86,8 @@ typedef struct cbspmv_opts {
     int stream_split;     /* enum cbspmv_split: scattered entries in a sweep kernel  */
 } cbspmv_opts;
 
-/* Scattered-entry split (FP64 / FP32 plans with native accumulation on the
- * stream schedule, no column panels).  An entry whose x line at most three
+/* Scattered-entry split (plans with native accumulation on the stream
+ * schedule, no column panels).  An entry whose x line at most three
  * entries among the ~nnz/tiles entries around it read (itself included)
  * costs the stream schedule a vector-L1 miss, and an L2 miss once x outgrows
  * one XCD's L2.  The split

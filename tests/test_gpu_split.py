@@ -1,7 +1,8 @@
 """Scattered-entry split on the GPU: y from the stream plan of the other
 entries plus the sweep kernel, checked against the oracle's FP64 SpMV of the
-matrix (FP64 plans: relative L2 <= 1e-12; FP32 plans: of the matrix with
-FP32-rounded values, <= 1e-6), on matrices with empty rows, rows whose
+matrix (FP64 plans: relative L2 <= 1e-12; FP32 and BF16 plans: of the
+matrix with the values rounded as stored, <= 1e-6), on matrices with empty
+rows, rows whose
 entries are all scattered, a row of 9000 scattered entries (a block alone),
 n of 2^25 (128-row blocks), and an all-scattered matrix (empty near plan).
 The sweep sums every row in a fixed order: repeated launches are identical.
@@ -53,7 +54,7 @@ def _x(n, seed, dtype):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("dtype", ["f64", "f32", "bf16"])
 @pytest.mark.parametrize("name", list(CASES))
 def test_split_spmv(gpu, name, dtype):
     import torch

@@ -121,8 +121,12 @@ def test_split_policy():
         p = Plan(m, n, row, col, val, split=split)
         assert (p.stats()["split_entries"] > 0) == expect, split
         p.close()
-    # neither BF16 nor compensated nor column-panel plans split
-    for opts in (dict(dtype="bf16"), dict(dtype="f32", compensated=True), dict(panels=2, tile_waves=8)):
+    # BF16 plans split (their sweep accumulates in FP32), compensated and
+    # column-panel plans do not
+    p = Plan(m, n, row, col, val, split="on", dtype="bf16")
+    assert p.stats()["split_entries"] > 0
+    p.close()
+    for opts in (dict(dtype="f32", compensated=True), dict(dtype="bf16", compensated=True), dict(panels=2, tile_waves=8)):
         p = Plan(m, n, row, col, val, split="on", **opts)
         assert p.stats()["split_entries"] == 0, opts
         p.close()
