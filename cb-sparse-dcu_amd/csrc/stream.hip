@@ -192,37 +192,67 @@ __device__ __forceinline__ f2 wave_shr1(f2 v) { return f2{dpp<kWaveShr1, 0xf>(v.
 // line-id loads, then all x loads, issued back to back (16 consecutive
 // threads read one line): two dependent round trips per pass, one pass for
 // the default cache; the last slot stays 0 for the padding entries.
+// mid() runs once, between the first pass's line-id loads and its x loads
+// (or alone when the tile stages nothing): the caller's wave issues its own
+// first loads there, so that their round trips overlap the staging's.
 // Every thread of the workgroup calls this; it ends with a workgroup barrier.
-template <typename V, typename A>
-__device__ __forceinline__ void stage_tile(const StreamArgs<V, A>& a, uint32_t tile, A* xs) {
-    // one pass for the default 96 KB cache and 16 waves: up to 24,576
-    // (FP32) slots over 1024 threads
-    constexpr int PER = 24;
+template <typename V, typename A, typename Mid>
+__device__ __forceinline__ void stage_tile(const StreamArgs<V, A>& a, uint32_t tile, A* xs, Mid&& mid) {
+    // one pass for the default 96 KB cache and 16 waves: up to 12,288 FP64
+    // or 24,576 FP32 slots over 1024 threads
+    constexpr int PER = 96 * 1024 / (int)sizeof(A) / kMaxStreamThreads;
     const uint32_t nthreads = blockDim.x;
     const uint32_t lb = a.tiles[2 * tile], cnt = a.tiles[2 * tile + 1] * kLineElems;
-    for (uint32_t s0 = threadIdx.x; s0 < cnt; s0 += PER * nthreads) {
+    const uint32_t kZero = a.tile_slots - 1;   // stays 0: read by the padding entries
+    bool pending = true;   // mid() not yet run (wave-uniform)
+    for (uint32_t p0 = 0; p0 < cnt; p0 += PER * nthreads) {
+        const uint32_t s0 = p0 + threadIdx.x;
+        // Every load is issued by every lane, from an index clamped into its
+        // array (a slot past the tile's reads its last line, a column past
+        // x reads x's last element; both are dropped), so that all of them
+        // are in flight together: behind a branch each would wait for the last.
         uint32_t lid[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const uint32_t s = s0 + j * nthreads;
-            lid[j] = s < cnt ? gptr(a.tile_lines)[lb + s / kLineElems] : 0u;
+            const uint32_t s = min(s0 + j * nthreads, cnt - 1);
+            lid[j] = gptr(a.tile_lines)[lb + s / kLineElems];
+        }
+        if (pending) {
+            mid();
+            pending = false;
         }
         A xv[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const uint32_t s = s0 + j * nthreads;
-            const uint32_t c = lid[j] * kLineElems + (s % kLineElems);
-            xv[j] = (s < cnt && c < (uint32_t)a.n) ? gptr(a.x)[c] : A(0);
+            const uint32_t c = lid[j] * kLineElems + ((s0 + j * nthreads) % kLineElems);
+            xv[j] = gptr(a.x)[min(c, (uint32_t)a.n - 1)];
         }
+        // every lane stores too (a slot past the tile writes the zero slot,
+        // past every staged one), else the compiler sinks a load into the store's branch
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const uint32_t s = s0 + j * nthreads;
-            if (s < cnt) xs[s] = xv[j];
+            const uint32_t c = lid[j] * kLineElems + (s % kLineElems);
+            xs[s < cnt ? s : kZero] = s < cnt && c < (uint32_t)a.n ? xv[j] : A(0);
         }
     }
-    if (threadIdx.x == 0) xs[a.tile_slots - 1] = A(0);
+    if (pending) mid();
+    if (threadIdx.x == 0) xs[kZero] = A(0);
     __syncthreads();
 }
+
+// stage_tile's mid() for a wave without loads of its own to overlap
+struct NoLoads {
+    __device__ __forceinline__ void operator()() const {}
+};
+
+// The stage() of an item that has no tile to stage (or one staged before):
+// its first loads are issued at once.
+struct RunNow {
+    template <typename F>
+    __device__ __forceinline__ void operator()(F&& f) const { f(); }
+};
+constexpr RunNow run_now{};
 
 // Pieces of a cut row: each piece's wave takes a ticket once its partial
 // sum is stored; the last piece to arrive sums them in piece order.
@@ -301,8 +331,9 @@ constexpr int kNearRing = 2;
 // One item with x in L2: chunk c + 1's keys and values are loaded (non-
 // temporal) while chunk c is reduced, then chunk c + 1's x is gathered; lanes
 // without a gather or a finished run skip the instruction.
-template <typename V, typename A, bool COMP, int EPL, bool SDDMM, bool LDS>
-__device__ __forceinline__ void stream_item_near(const StreamArgs<V, A>& a, uint32_t item, const A* xs, A* yout) {
+template <typename V, typename A, bool COMP, int EPL, bool SDDMM, bool LDS, typename Stage>
+__device__ __forceinline__ void stream_item_near(const StreamArgs<V, A>& a, uint32_t item, const A* xs, A* yout,
+                                                 Stage&& stage) {
     using Acc = typename std::conditional<COMP, f2, A>::type;
     const uint32_t kZero = a.tile_slots - 1;
     const int lane = lane_id();
@@ -310,8 +341,15 @@ __device__ __forceinline__ void stream_item_near(const StreamArgs<V, A>& a, uint
     const int cb = a.col_bits;
     const uint32_t cmask = (1u << cb) - 1u;
     const uint32_t omask = (1u << (31 - cb)) - 1u;
-    const uint32_t split_first = it.first_partial ? gptr(a.item_split_first)[item] : 0xffffffffu;
-    const uint32_t split_last = it.last_partial ? gptr(a.item_split)[item] : 0xffffffffu;
+    // The cut-row tables are read at once when the item has no tile to stage
+    // (the registers are shorter then), else after the staging, which must
+    // not wait for this item's descriptor.
+    constexpr bool kLate = !std::is_same<std::decay_t<Stage>, RunNow>::value;
+    uint32_t split_first = 0xffffffffu, split_last = 0xffffffffu;
+    if constexpr (!kLate) {
+        if (it.first_partial) split_first = gptr(a.item_split_first)[item];
+        if (it.last_partial) split_last = gptr(a.item_split)[item];
+    }
     auto put_partial = [&](uint32_t slot, Acc v) {
         if constexpr (COMP) {
             gptr(a.partial)[2 * slot] = v.hi;
@@ -353,7 +391,11 @@ __device__ __forceinline__ void stream_item_near(const StreamArgs<V, A>& a, uint
     Acc carry{};
     Chunk<V, EPL> cur, nxt;
     A xc[EPL];
-    cur.load(a.slab, it.e_begin, lanes_of(it.e_begin), lane);
+    stage([&] { cur.load(a.slab, it.e_begin, lanes_of(it.e_begin), lane); });
+    if constexpr (kLate) {
+        if (it.first_partial) split_first = gptr(a.item_split_first)[item];
+        if (it.last_partial) split_last = gptr(a.item_split)[item];
+    }
     gather(cur.k, xc);
     for (uint32_t e0 = it.e_begin; e0 < it.e_end; e0 += kChunk) {
         const bool more = e0 + kChunk < it.e_end;
@@ -425,10 +467,11 @@ __device__ __forceinline__ void stream_item_near(const StreamArgs<V, A>& a, uint
 // One item, streamed by one wave: y rows go to yout (y, or a panel's partial
 // sums).  With LDS, keys flagged as staged read x from the tile's LDS cache
 // xs; without (panel items), such keys are padding and read 0.
-template <typename V, typename A, bool COMP, int EPL, bool SDDMM, bool LDS, int RING>
-__device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t item, const A* xs, A* yout) {
+template <typename V, typename A, bool COMP, int EPL, bool SDDMM, bool LDS, int RING, typename Stage>
+__device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t item, const A* xs, A* yout,
+                                            Stage&& stage) {
     if constexpr (RING == kNearRing) {
-        stream_item_near<V, A, COMP, EPL, SDDMM, LDS>(a, item, xs, yout);
+        stream_item_near<V, A, COMP, EPL, SDDMM, LDS>(a, item, xs, yout, stage);
         return;
     }
     using Acc = typename std::conditional<COMP, f2, A>::type;
@@ -438,8 +481,15 @@ __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t 
     const int cb = a.col_bits;
     const uint32_t cmask = (1u << cb) - 1u;
     const uint32_t omask = (1u << (31 - cb)) - 1u;
-    const uint32_t split_first = it.first_partial ? gptr(a.item_split_first)[item] : 0xffffffffu;
-    const uint32_t split_last = it.last_partial ? gptr(a.item_split)[item] : 0xffffffffu;
+    // The cut-row tables are read at once when the item has no tile to stage
+    // (the registers are shorter then), else after the staging, which must
+    // not wait for this item's descriptor.
+    constexpr bool kLate = !std::is_same<std::decay_t<Stage>, RunNow>::value;
+    uint32_t split_first = 0xffffffffu, split_last = 0xffffffffu;
+    if constexpr (!kLate) {
+        if (it.first_partial) split_first = gptr(a.item_split_first)[item];
+        if (it.last_partial) split_last = gptr(a.item_split)[item];
+    }
 
     // A finished run (row offset `off`) goes to y, or to the partial buffer
     // when it is a piece of a cut row (the item's first run, offset 0, or its
@@ -464,12 +514,13 @@ __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t 
     static_assert(RING == kNearRing || RING >= 3, "the ring holds the reduced, the gathered and a loading chunk");
     constexpr uint32_t EB = 4 + sizeof(V);
     constexpr uint32_t kChunk = kWave * EPL;
-    const uint32_t ne = it.e_end - it.e_begin;
-    const rsrc_t rs = make_rsrc(a.slab + (size_t)it.e_begin * EB, ne * EB);
+    // Set by stage()'s hook (nothing that waits for the item's descriptor
+    // may come before the staging's first loads) and after it.
+    uint32_t ne = 0;
+    rsrc_t rs = make_rsrc(a.slab, 0);
     const rsrc_t rx = make_rsrc(a.x, (uint32_t)a.n * (uint32_t)sizeof(A));
-    const uint32_t item_rows = min((uint32_t)a.m - it.row_base, omask + 1u);
-    const rsrc_t ry = make_rsrc(yout + it.row_base, item_rows * (uint32_t)sizeof(A));
-    const bool first_piece = it.first_partial != 0;
+    rsrc_t ry = make_rsrc(yout, 0);
+    bool first_piece = false;
 
     auto gather = [&](const LaneChunk<V, EPL>& c, A* g, bool live) {
 #pragma unroll
@@ -554,8 +605,19 @@ __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t 
 
     LaneChunk<V, EPL> c[RING];
     A g[RING][EPL];
+    // The first RING - 1 chunks are loaded while the tile's x lines are staged.
+    stage([&] {
+        ne = it.e_end - it.e_begin;
+        rs = make_rsrc(a.slab + (size_t)it.e_begin * EB, ne * EB);
 #pragma unroll
-    for (int d = 0; d < RING - 1; ++d) c[d].load(rs, d * kChunk, ne, lane);
+        for (int d = 0; d < RING - 1; ++d) c[d].load(rs, d * kChunk, ne, lane);
+    });
+    ry = make_rsrc(yout + it.row_base, min((uint32_t)a.m - it.row_base, omask + 1u) * (uint32_t)sizeof(A));
+    first_piece = it.first_partial != 0;
+    if constexpr (kLate) {
+        if (it.first_partial) split_first = gptr(a.item_split_first)[item];
+        if (it.last_partial) split_last = gptr(a.item_split)[item];
+    }
     gather(c[0], g[0], true);
     uint32_t r0 = 0;
     for (bool more = true; more;) {
@@ -605,18 +667,33 @@ __device__ __forceinline__ void stream_item(const StreamArgs<V, A>& a, uint32_t 
 // Every thread of the workgroup calls this.
 template <typename V, typename A, bool COMP, int EPL, bool SDDMM, int RING, bool LOOP = false>
 __device__ __forceinline__ void stream_tile(const StreamArgs<V, A>& a, uint32_t tile, A* xs) {
-    stage_tile(a, tile, xs);
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // wave-uniform
     if constexpr (LOOP) {
+        stage_tile(a, tile, xs, NoLoads{});
         const uint32_t waves = blockDim.x >> 6;
         for (uint32_t w = wave; w < a.tile_items; w += waves) {
             const uint32_t item = tile * a.tile_items + w;
             if (item >= a.n_items) break;
-            stream_item<V, A, COMP, EPL, SDDMM, true, RING>(a, item, xs, a.y);
+            stream_item<V, A, COMP, EPL, SDDMM, true, RING>(a, item, xs, a.y, run_now);
         }
     } else {
+        // A wave with an item stages the tile from inside the item, after
+        // its first slab loads are issued; the others stage it here.  Every
+        // wave reaches the staging's barrier once.  Not with 8-entry lanes
+        // in the ring, whose registers are short already: their waves stage
+        // the tile first.
         const uint32_t item = tile * a.tile_items + wave;
-        if (wave < a.tile_items && item < a.n_items) stream_item<V, A, COMP, EPL, SDDMM, true, RING>(a, item, xs, a.y);
+        const bool has_item = wave < a.tile_items && item < a.n_items;
+        if constexpr (RING == kNearRing || EPL == 4) {
+            if (has_item)
+                stream_item<V, A, COMP, EPL, SDDMM, true, RING>(a, item, xs, a.y,
+                                                                [&](auto&& mid) { stage_tile(a, tile, xs, mid); });
+            else
+                stage_tile(a, tile, xs, NoLoads{});
+        } else {
+            stage_tile(a, tile, xs, NoLoads{});
+            if (has_item) stream_item<V, A, COMP, EPL, SDDMM, true, RING>(a, item, xs, a.y, run_now);
+        }
     }
 }
 
@@ -894,7 +971,7 @@ __global__ __launch_bounds__(kMaxStreamThreads) void stream_spmv(StreamArgs<V, A
     A* out = a.panel_out + (size_t)q * (uint32_t)a.m;
     for (uint32_t i = a.panel_item_ptr[q] + rank * a.tile_items + (wave - a.tile_items); i < a.panel_item_ptr[q + 1];
          i += stride)
-        stream_item<V, A, false, EPL, SDDMM, false, RING>(a, i, nullptr, out);
+        stream_item<V, A, false, EPL, SDDMM, false, RING>(a, i, nullptr, out, run_now);
 }
 
 // y[r] += the panel sums of row r (in panel order), for rows with panel
