@@ -52,6 +52,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_spmv(const A* __restrict_
     const uint32_t cmask = (1u << col_bits) - 1u;
     for (int i = (int)threadIdx.x; i < kSweepMaxRows; i += kSweepThreads) yt[i] = A(0);
     const uint32_t e0 = gptr(wave_off)[b * kSweepWaves + wave], e1 = gptr(wave_off)[b * kSweepWaves + wave + 1];
+    const int r0 = gptr(blk_row)[b], span = gptr(blk_span)[b];   // span >= 1
     uint32_t kn[U];
     A vn[U];
 #pragma unroll
@@ -88,8 +89,22 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_spmv(const A* __restrict_
         }
     }
     __syncthreads();
-    const int r0 = gptr(blk_row)[b], span = gptr(blk_span)[b];
-    for (int i = (int)threadIdx.x; i < span; i += kSweepThreads) gptr(y)[r0 + i] += yt[i];
+    // y[rows] += the row tile: every y load of the thread is issued first
+    // (indices clamped to the block's last row), and each value passes an
+    // empty asm, which keeps the compiler from sinking a load into its
+    // store's branch; a loop of load, wait and store would pay one round
+    // trip per 256 rows.
+    constexpr int R = kSweepMaxRows / kSweepThreads;
+    A yv[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) yv[k] = gptr(y)[r0 + min((int)threadIdx.x + k * kSweepThreads, span - 1)];
+#pragma unroll
+    for (int k = 0; k < R; ++k) __asm__ volatile("" : "+v"(yv[k]));
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = (int)threadIdx.x + k * kSweepThreads;
+        if (i < span) gptr(y)[r0 + i] = yv[k] + yt[i];
+    }
 }
 
 }  // namespace dev
