@@ -64,6 +64,7 @@ struct cbspmv_plan {
     // scattered ones (cbspmv_spmv runs `near`, then the sweep kernel)
     cbspmv_plan* near = nullptr;
     cbs::SweepPart sweep;
+    bool paced = false;   // this plan is another's `near` (StreamArgs::paced)
 };
 
 namespace cbs {

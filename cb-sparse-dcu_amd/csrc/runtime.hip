@@ -194,6 +194,7 @@ int run_stream(cbspmv_plan& p, const void* x, void* y, hipStream_t s, const void
         a.n = h.n;
         a.panels = (uint32_t)h.panels;
         a.panel_item_ptr = static_cast<const uint32_t*>(d.panel_items.ptr);
+        a.paced = p.paced ? 1u : 0u;
     };
     const bool panels = h.panels > 1;
     const uint8_t* mask = static_cast<const uint8_t*>(d.panel_mask.ptr);

@@ -71,6 +71,7 @@ int build_split(cbspmv_plan& p, int nnz, const int* row, const int* col, const d
     cbspmv_plan* near = nullptr;
     const int rc = build_plan_object(o, h.m, h.n, (int)nr.size(), nr.data(), nc.data(), nv.data(), false, &near);
     if (rc != CBSPMV_OK) return rc;
+    near->paced = true;
     p.near = near;
     p.sweep = sweep_layout(fe, row, col, val, cb, max_rows);
     return CBSPMV_OK;

@@ -52,6 +52,10 @@ struct StreamArgs {
     uint32_t panels;
     const uint32_t* panel_item_ptr;  // panel q's items: [ptr[q], ptr[q + 1])
     A* panel_out;                    // panels x m partial sums
+    // 1 on the other entries' plan of a scattered-entry split: a wave's issue
+    // priority falls as its item progresses (ring loop), so the tile's
+    // lagging waves go first and its waves end closer together
+    uint32_t paced;
 };
 
 // One plan of a batched launch: its stream arguments (x, y, partial and
