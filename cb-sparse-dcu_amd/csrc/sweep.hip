@@ -38,7 +38,7 @@ __device__ __forceinline__ float sweep_value<float, uint16_t>(const uint16_t* v,
     return __uint_as_float((uint32_t)gptr(v)[k] << 16);
 }
 
-template <typename A, typename V>
+template <typename A, typename V, bool PACED>
 __global__ __launch_bounds__(kSweepThreads) void sweep_spmv(const A* __restrict__ x, const uint32_t* __restrict__ key,
                                                             const V* __restrict__ val,
                                                             const uint32_t* __restrict__ wave_off,
@@ -62,7 +62,27 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_spmv(const A* __restrict_
         vn[u] = k < e1 ? sweep_value<A>(val, k) : A(0);
     }
     __syncthreads();
+    // Paced (x beyond 8 MiB): a wave's priority starts at 3 and falls by one
+    // each quarter of its entries, which keeps the waves of a CU at one pace
+    // through x (the 2,097,152^2 block-clustered matrix, FP64, x 16 MiB:
+    // 194.0 -> 186.9 us per split SpMV; the 1,048,576^2 stencil / power-law
+    // one, x 8 MiB: 124.0 -> 127.7 us, so not there).
+    [[maybe_unused]] const uint32_t quarter = (e1 - e0) / 4;
+    [[maybe_unused]] int pace = 0;
+    if constexpr (PACED) __builtin_amdgcn_s_setprio(3);
     for (uint32_t k0 = e0; k0 < e1; k0 += U * kWave) {
+        if constexpr (PACED) {
+            if (pace == 0 && k0 - e0 >= quarter) {
+                __builtin_amdgcn_s_setprio(2);
+                pace = 1;
+            } else if (pace == 1 && k0 - e0 >= 2 * quarter) {
+                __builtin_amdgcn_s_setprio(1);
+                pace = 2;
+            } else if (pace == 2 && k0 - e0 >= 3 * quarter) {
+                __builtin_amdgcn_s_setprio(0);
+                pace = 3;
+            }
+        }
         uint32_t kk[U];
         A v[U], xv[U];
 #pragma unroll
@@ -88,6 +108,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_spmv(const A* __restrict_
             if (valid && (lane == kWave - 1 || nrow != row)) yt[row] += s;
         }
     }
+    if constexpr (PACED) __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     // y[rows] += the row tile: every y load of the thread is issued first
     // (indices clamped to the block's last row), and each value passes an
@@ -120,16 +141,18 @@ int run_sweep(cbspmv_plan& p, const void* x, void* y, hipStream_t s) {
     const auto* wo = static_cast<const uint32_t*>(d.sw_wave_off.ptr);
     const auto* br = static_cast<const int32_t*>(d.sw_blk_row.ptr);
     const auto* bs = static_cast<const int32_t*>(d.sw_blk_span.ptr);
+    const size_t xb = (size_t)p.host.n * (p.host.o.dtype == CBSPMV_F64 ? 8 : 4);
+    const bool paced = xb > (size_t(8) << 20);
     if (p.host.o.dtype == CBSPMV_F64)
-        hipLaunchKernelGGL((dev::sweep_spmv<double, double>), dim3(blocks), dim3(dev::kSweepThreads), 0, s,
+        hipLaunchKernelGGL((paced ? dev::sweep_spmv<double, double, true> : dev::sweep_spmv<double, double, false>), dim3(blocks), dim3(dev::kSweepThreads), 0, s,
                            static_cast<const double*>(x), key, static_cast<const double*>(d.sw_val.ptr), wo, br, bs,
                            (uint32_t)sw.col_bits, static_cast<double*>(y));
     else if (p.host.o.dtype == CBSPMV_F32)
-        hipLaunchKernelGGL((dev::sweep_spmv<float, float>), dim3(blocks), dim3(dev::kSweepThreads), 0, s,
+        hipLaunchKernelGGL((paced ? dev::sweep_spmv<float, float, true> : dev::sweep_spmv<float, float, false>), dim3(blocks), dim3(dev::kSweepThreads), 0, s,
                            static_cast<const float*>(x), key, static_cast<const float*>(d.sw_val.ptr), wo, br, bs,
                            (uint32_t)sw.col_bits, static_cast<float*>(y));
     else   // BF16 values, FP32 vectors
-        hipLaunchKernelGGL((dev::sweep_spmv<float, uint16_t>), dim3(blocks), dim3(dev::kSweepThreads), 0, s,
+        hipLaunchKernelGGL((paced ? dev::sweep_spmv<float, uint16_t, true> : dev::sweep_spmv<float, uint16_t, false>), dim3(blocks), dim3(dev::kSweepThreads), 0, s,
                            static_cast<const float*>(x), key, static_cast<const uint16_t*>(d.sw_val.ptr), wo, br, bs,
                            (uint32_t)sw.col_bits, static_cast<float*>(y));
     const hipError_t e = hipGetLastError();
